@@ -1,0 +1,47 @@
+"""Build recipe for libgqsort.so (gfx950) -- run on any host with hipcc.
+
+    python gpu-quicksort_amd/build.py            # build the library in-tree
+
+The library is written to gpu-quicksort_amd/gqsort/libgqsort.so next to the
+Python binding that loads it.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "gqsort", "libgqsort.so")
+ARCH = os.environ.get("GQS_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["kernels.hip", "gqsort.cpp", "generate.cpp", "batch.cpp", "serve.cpp"]
+
+# Packed FP32 instructions are disabled for device code; the host pass of
+# hipcc warns that it does not know the feature, which is harmless.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.isabs(cand) and os.path.exists(cand) or not os.path.isabs(cand)):
+            return cand
+    return "hipcc"
+
+
+def build(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
+           "-Wno-unused-command-line-argument", *DEVICE_FLAGS,
+           "-I", os.path.join(HERE, "..", "include"), "-o", OUT + ".tmp", *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libgqsort.so")
+    os.replace(OUT + ".tmp", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

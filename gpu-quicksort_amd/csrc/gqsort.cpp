@@ -1,0 +1,470 @@
+// gqsort.cpp -- host dispatcher and C ABI of the GPU-Quicksort library.
+//
+// GPUQSort's host side (CL12/QuicksortMain.cpp:297-384) builds per-level
+// block/parent records, launches gqsort_kernel, reads the results back and
+// splits them into work/done, once per level, then launches lqsort_kernel.
+// Here the whole schedule is enqueued up front: each level's bookkeeping runs
+// on the device (kernels.hip), so a sort of any size is one stream-ordered
+// chain with no host round trip.  `host_driven` restores the reference's
+// loop shape (one readback per level, stop when the queue is empty), which
+// the classic single-pivot mode uses by default because its level count is
+// data dependent (about 20 at 2^28 keys).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gqsort.h"
+#include "internal.h"
+#include "kernels.h"
+
+using namespace gqs;
+
+namespace {
+
+size_t key_bytes(int dtype) { return dtype == DT_F64 ? 8 : 4; }
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+}  // namespace
+
+namespace gqs {
+
+bool Ctx::fail(hipError_t e, const char* what) {
+    if (e == hipSuccess) return false;
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    err = buf;
+    return true;
+}
+
+gqs_status Ctx::set_error(gqs_status s, const char* msg) {
+    err = msg;
+    return s;
+}
+
+void Workspace::release() {
+    hipFree(segs[0]);
+    hipFree(segs[1]);
+    hipFree(plans);
+    hipFree(spl);
+    hipFree(meta);
+    hipFree(mat);
+    hipFree(part);
+    hipFree(bkt);
+    hipFree(ctl);
+    hipFree(leaves);
+    *this = Workspace{};
+}
+
+// Capacities from provable bounds (n = total keys, S = segments per level):
+//   every segment of a level holds > leaf_max keys  =>  S <= n/(leaf_max+1)
+//   multipivot fan-out k <= max(2, 2*ceil(n_s/leaf_target)), buckets <= 2k-1
+//   count matrix <= sum_s ntiles_s * nbuckets_s
+//               <= 256*(n/tile) + sum_s nbuckets_s
+Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
+    Caps c;
+    const uint64_t tile = (dtype == DT_F64) ? TileShape<uint64_t>::kTileKeys : TileShape<uint32_t>::kTileKeys;
+    c.seg = n / ((uint64_t)p.leaf_max + 1) + 2;
+    const uint64_t lt = std::max<uint64_t>(1, p.leaf_target);
+    const uint64_t sum_nb = (p.mode == MODE_CLASSIC) ? 3 * c.seg : 8 * c.seg + 4 * (n / lt + 1);
+    c.bkt = sum_nb + 256;
+    c.mat = 256 * (n / tile + 1) + sum_nb + 256;
+    c.part = c.mat / kScanChunk + 2;
+    c.leaves = std::max<uint64_t>(1, nseg_in);
+    return c;
+}
+
+gqs_status Ctx::ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
+    const Caps need = caps_for(n, nseg_in, p, dtype);
+    const size_t kb = key_bytes(dtype);
+    if (ws.ctl && need.seg <= ws.cap.seg && need.mat <= ws.cap.mat && need.bkt <= ws.cap.bkt &&
+        need.part <= ws.cap.part && need.leaves <= ws.cap.leaves && kb <= ws.key_bytes)
+        return GQS_OK;
+    if (ws.ctl) {
+        hipStreamSynchronize(last_stream);
+        ws.release();
+    }
+    Caps c = need;
+    // Grow with headroom so that nearby sizes reuse the workspace.
+    c.seg += c.seg / 4;
+    c.mat += c.mat / 4;
+    c.bkt += c.bkt / 4;
+    c.part += c.part / 4 + 1;
+    hipError_t e = hipSuccess;
+    if (!e) e = dalloc(&ws.segs[0], c.seg);
+    if (!e) e = dalloc(&ws.segs[1], c.seg);
+    if (!e) e = dalloc(&ws.plans, c.seg);
+    if (!e) e = hipMalloc(&ws.spl, std::max<size_t>(1, c.seg * kSplStride * 8));
+    if (!e) e = dalloc(&ws.meta, c.seg * kSplStride);
+    if (!e) e = dalloc(&ws.mat, c.mat);
+    if (!e) e = dalloc(&ws.part, c.part);
+    if (!e) e = dalloc(&ws.bkt, c.bkt);
+    if (!e) e = dalloc(&ws.ctl, (size_t)kMaxLevels + 1);
+    if (!e) e = dalloc(&ws.leaves, c.leaves);
+    if (e) {
+        ws.release();
+        fail(e, "workspace allocation");
+        return GQS_ENOMEM;
+    }
+    ws.cap = c;
+    ws.key_bytes = 8;
+    return GQS_OK;
+}
+
+Grid Ctx::grid() const {
+    Grid g;
+    g.part = cus * (int)params.part_wgs_per_cu;
+    g.splitters = std::min(1024, cus * 4);
+    g.scan = cus * 2;
+    g.plan = cus * 2;
+    g.leaf = cus * (int)params.leaf_wgs_per_cu;
+    return g;
+}
+
+void Ctx::prof_begin(int kind, hipStream_t st) {
+    if (!profiling) return;
+    ProfRec r;
+    r.kind = kind;
+    hipEventCreate(&r.a);
+    hipEventCreate(&r.b);
+    hipEventRecord(r.a, st);
+    prof.push_back(r);
+}
+void Ctx::prof_end(hipStream_t st) {
+    if (!profiling || prof.empty()) return;
+    hipEventRecord(prof.back().b, st);
+}
+
+uint32_t levels_needed(uint64_t n, const gqs_params& p) {
+    if (n <= p.leaf_max) return 0;
+    uint32_t L = 0;
+    double sz = (double)n;
+    const double shrink = (p.mode == MODE_CLASSIC) ? 1.5 : std::max(2.0, p.max_fanout / 2.0);
+    while (sz > p.leaf_max && L < (uint32_t)kMaxLevels - 2) {
+        sz = sz / shrink;
+        ++L;
+    }
+    return std::max<uint32_t>(1, std::min<uint32_t>(L + 1, std::min<uint32_t>(p.max_levels, kMaxLevels - 1)));
+}
+
+// The whole sort as one stream-ordered chain.  Either a single key range
+// [0, n) of d (offsets == nullptr) or nseg arrays delimited by offsets.
+gqs_status Ctx::run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets, uint32_t nseg_in,
+                    hipStream_t st) {
+    gqs_params p = params;
+    const uint32_t cap = (dtype == DT_F64) ? 8192u : 16384u;
+    p.leaf_max = std::min(p.leaf_max, cap);
+    if (gqs_status s = ensure_workspace(n, nseg_in, p, dtype)) return s;
+    last_stream = st;
+    if (profiling) {
+        for (auto& r : prof) {
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+        prof.clear();
+    }
+    const Grid g = grid();
+    uint32_t L;
+    if (offsets) {
+        L = levels_needed(n, p);
+        prof_begin(7, st);
+        if (fail(launch_init(ws.ctl, kMaxLevels, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
+        if (fail(launch_seed_segments(offsets, nseg_in, ws.segs[0], &ws.ctl[0], ws.leaves, &ws.ctl[0].nleaf_list,
+                                      p.leaf_max, st),
+                 "seed"))
+            return GQS_EDEVICE;
+        prof_end(st);
+    } else {
+        const bool as_leaf = n <= p.leaf_max;
+        L = as_leaf ? 0 : levels_needed(n, p);
+        prof_begin(7, st);
+        if (fail(launch_init(ws.ctl, kMaxLevels, as_leaf ? ws.leaves : ws.segs[0], 1, n,
+                             SEG_RAW | (as_leaf ? 0x100u : 0u), st),
+                 "init"))
+            return GQS_EDEVICE;
+        prof_end(st);
+    }
+    LevelArgs a{};
+    a.d = d;
+    a.dn = dn;
+    a.plans = ws.plans;
+    a.spl = ws.spl;
+    a.meta = ws.meta;
+    a.mat = ws.mat;
+    a.part = ws.part;
+    a.bkt = ws.bkt;
+    a.mode = (uint32_t)p.mode;
+    a.leaf_max = p.leaf_max;
+    a.leaf_target = p.leaf_target;
+    a.oversample = p.oversample;
+    a.max_fanout = p.max_fanout;
+    a.seg_cap = (uint32_t)ws.cap.seg;
+    a.mat_cap = (uint32_t)ws.cap.mat;
+    a.bkt_cap = (uint32_t)ws.cap.bkt;
+    a.seed = 0x5EEDu;
+    // Level-0 leaves: the whole input when it fits one local sort, or the
+    // short arrays of a segmented sort (all still in caller representation).
+    if (offsets || L == 0) {
+        LeafArgs la{};
+        la.d = d;
+        la.dn = dn;
+        la.list = ws.leaves;
+        la.list_count = &ws.ctl[0].nleaf_list;
+        la.lv = a;
+        la.lv.ctl = &ws.ctl[0];
+        la.leaf_max = p.leaf_max;
+        la.is_last = 1;
+        prof_begin(6, st);
+        if (fail(launch_leaf(dtype, la, g, st), "leaf(list)")) return GQS_EDEVICE;
+        prof_end(st);
+    }
+    stats = gqs_stats{};
+    stats.levels_launched = L;
+    for (uint32_t lvl = 0; lvl < L; ++lvl) {
+        a.segs = ws.segs[lvl & 1];
+        a.next_segs = ws.segs[(lvl + 1) & 1];
+        a.ctl = &ws.ctl[lvl];
+        a.ctl_next = &ws.ctl[lvl + 1];
+        a.is_last = (lvl + 1 == L) ? 1u : 0u;
+        a.seed = 0x5EEDu + 977u * lvl;
+        for (int step = 0; step < kLevelSteps; ++step) {
+            prof_begin(step_kind(step), st);
+            if (fail(launch_level_step(step, dtype, a, g, st), "level")) return GQS_EDEVICE;
+            prof_end(st);
+        }
+        LeafArgs la{};
+        la.d = d;
+        la.dn = dn;
+        la.list = nullptr;
+        la.list_count = nullptr;
+        la.lv = a;
+        la.leaf_max = p.leaf_max;
+        la.is_last = a.is_last;
+        prof_begin(6, st);
+        if (fail(launch_leaf(dtype, la, g, st), "leaf")) return GQS_EDEVICE;
+        prof_end(st);
+        if (p.host_driven && !a.is_last) {
+            uint32_t next = 0;
+            if (fail(hipMemcpyAsync(&next, &ws.ctl[lvl + 1].nseg, sizeof next, hipMemcpyDeviceToHost, st),
+                     "level readback") ||
+                fail(hipStreamSynchronize(st), "level sync"))
+                return GQS_EDEVICE;
+            stats.levels_used = lvl + 1;
+            if (next == 0) break;
+        }
+    }
+    return GQS_OK;
+}
+
+gqs_status Ctx::fetch_stats() {
+    if (!ws.ctl) return GQS_OK;
+    std::vector<LevelCtl> c(kMaxLevels);
+    if (fail(hipStreamSynchronize(last_stream), "sync") ||
+        fail(hipMemcpy(c.data(), ws.ctl, sizeof(LevelCtl) * kMaxLevels, hipMemcpyDeviceToHost), "stats readback"))
+        return GQS_EDEVICE;
+    uint32_t used = 0, over = 0, ovf = 0;
+    for (int i = 0; i < kMaxLevels; ++i) {
+        if (c[i].nseg) used = i + 1;
+        over += c[i].oversize;
+        ovf |= c[i].overflow;
+        if (i < 8) stats.segments[i] = c[i].nseg;
+    }
+    stats.levels_used = std::max(stats.levels_used, used);
+    stats.oversize_leaves = over;
+    stats.overflow = ovf;
+    return GQS_OK;
+}
+
+}  // namespace gqs
+
+// ================================================================== C ABI
+extern "C" {
+
+gqs_status gqs_default_params(int dtype, gqs_params* out) {
+    if (!out) return GQS_EINVAL;
+    gqs_params p{};
+    p.mode = GQS_MODE_MULTIPIVOT;
+    p.leaf_max = (dtype == DT_F64) ? 4096 : 8192;
+    p.leaf_target = (dtype == DT_F64) ? 2048 : 4096;
+    p.oversample = 16;
+    p.max_fanout = 256;
+    p.max_levels = 8;
+    p.host_driven = 0;
+    p.part_wgs_per_cu = 4;
+    p.leaf_wgs_per_cu = 4;
+    *out = p;
+    return GQS_OK;
+}
+
+gqs_status gqs_create(gqs_ctx** out, int device) {
+    if (!out) return GQS_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return GQS_EDEVICE;
+    auto* c = new (std::nothrow) gqs_ctx();
+    if (!c) return GQS_ENOMEM;
+    c->device = device;
+    hipDeviceProp_t prop{};
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete c;
+        return GQS_EDEVICE;
+    }
+    c->cus = prop.multiProcessorCount;
+    c->lds_per_wg = (int)prop.sharedMemPerBlock;
+    c->wave = prop.warpSize;
+    c->name = prop.gcnArchName;
+    gqs_default_params(DT_U32, &c->params);
+    *out = c;
+    return GQS_OK;
+}
+
+gqs_status gqs_destroy(gqs_ctx* c) {
+    if (!c) return GQS_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        hipSetDevice(c->device);
+        hipDeviceSynchronize();
+        for (auto& r : c->prof) {
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+        c->ws.release();
+        hipFree(c->host_d);
+        hipFree(c->host_dn);
+    }
+    delete c;
+    return GQS_OK;
+}
+
+const char* gqs_last_error(const gqs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+gqs_status gqs_get_params(const gqs_ctx* c, gqs_params* out) {
+    if (!c || !out) return GQS_EINVAL;
+    *out = c->params;
+    return GQS_OK;
+}
+
+gqs_status gqs_set_params(gqs_ctx* c, const gqs_params* p) {
+    if (!c || !p) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const bool pow2 = p->max_fanout >= 2 && p->max_fanout <= 256 && (p->max_fanout & (p->max_fanout - 1)) == 0;
+    if ((p->mode != GQS_MODE_MULTIPIVOT && p->mode != GQS_MODE_CLASSIC) || p->leaf_max < 16 ||
+        p->leaf_max > 16384 || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
+        p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32)
+        return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");
+    c->params = *p;
+    return GQS_OK;
+}
+
+gqs_status gqs_sort_device(gqs_ctx* c, int dtype, void* d, void* dn, size_t n, void* stream) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_sort_device: bad dtype");
+    if (n < 2) return GQS_OK;
+    if (!d || !dn) return c->set_error(GQS_EINVAL, "gqs_sort_device: null buffer");
+    if (n >= 0xFFFFFFFFull) return c->set_error(GQS_EINVAL, "gqs_sort_device: n must be < 2^32");
+    if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+    return c->run(dtype, d, dn, n, nullptr, 0, (hipStream_t)stream);
+}
+
+gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
+                              size_t total_keys, void* stream) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: bad dtype");
+    if (nseg == 0 || total_keys == 0) return GQS_OK;
+    if (!d || !dn || !offsets) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: null buffer");
+    if (nseg >= 0xFFFFFFFFull) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: too many arrays");
+    if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+    return c->run(dtype, d, dn, total_keys, offsets, (uint32_t)nseg, (hipStream_t)stream);
+}
+
+gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
+    if (!c) return GQS_EINVAL;
+    if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_sort: bad dtype");
+    if (n < 2) return GQS_OK;
+    if (!d || !dn) return c->set_error(GQS_EINVAL, "gqs_sort: null buffer");
+    const size_t bytes = n * key_bytes(dtype);
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+        if (bytes > c->host_cap) {
+            hipFree(c->host_d);
+            hipFree(c->host_dn);
+            c->host_d = c->host_dn = nullptr;
+            c->host_cap = 0;
+            if (hipMalloc(&c->host_d, bytes) != hipSuccess || hipMalloc(&c->host_dn, bytes) != hipSuccess) {
+                hipFree(c->host_d);
+                c->host_d = nullptr;
+                return c->set_error(GQS_ENOMEM, "gqs_sort: device buffers");
+            }
+            c->host_cap = bytes;
+        }
+        if (c->fail(hipMemcpy(c->host_d, d, bytes, hipMemcpyHostToDevice), "H2D")) return GQS_EDEVICE;
+    }
+    gqs_status s = gqs_sort_device(c, dtype, c->host_d, c->host_dn, n, nullptr);
+    if (s) return s;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->fail(hipMemcpy(d, c->host_d, bytes, hipMemcpyDeviceToHost), "D2H")) return GQS_EDEVICE;
+    return GQS_OK;
+}
+
+gqs_status gqs_get_stats(gqs_ctx* c, gqs_stats* out) {
+    if (!c || !out) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (gqs_status s = c->fetch_stats()) return s;
+    *out = c->stats;
+    return GQS_OK;
+}
+
+gqs_status gqs_set_profiling(gqs_ctx* c, int on) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->profiling = on != 0;
+    return GQS_OK;
+}
+
+gqs_status gqs_kernel_times(gqs_ctx* c, uint32_t* launches, double* ms, int nkinds) {
+    if (!c || !launches || !ms || nkinds <= 0) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int k = 0; k < nkinds; ++k) {
+        launches[k] = 0;
+        ms[k] = 0.0;
+    }
+    if (c->fail(hipStreamSynchronize(c->last_stream), "sync")) return GQS_EDEVICE;
+    for (auto& r : c->prof) {
+        float t = 0.f;
+        if (r.kind < nkinds && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            launches[r.kind]++;
+            ms[r.kind] += t;
+        }
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    c->prof.clear();
+    return GQS_OK;
+}
+
+gqs_status gqs_device_info(gqs_ctx* c, int* cus, int* lds_per_wg, int* wave_size, char* name, size_t name_len) {
+    if (!c) return GQS_EINVAL;
+    if (cus) *cus = c->cus;
+    if (lds_per_wg) *lds_per_wg = c->lds_per_wg;
+    if (wave_size) *wave_size = c->wave;
+    if (name && name_len) {
+        std::strncpy(name, c->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return GQS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// internal.h -- context and workspace of the GPU-Quicksort library (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gqsort.h"
+#include "kernels.h"
+#include "records.h"
+
+namespace gqs {
+
+struct Caps {
+    uint64_t seg = 0, mat = 0, bkt = 0, part = 0, leaves = 0;
+};
+
+// Device workspace, grown on demand and reused across sorts.
+struct Workspace {
+    SegRec* segs[2] = {nullptr, nullptr};  // level queues (ping-pong)
+    SegPlan* plans = nullptr;
+    void* spl = nullptr;
+    uint16_t* meta = nullptr;
+    uint32_t* mat = nullptr;
+    uint32_t* part = nullptr;
+    BucketRec* bkt = nullptr;
+    LevelCtl* ctl = nullptr;
+    SegRec* leaves = nullptr;
+    Caps cap;
+    size_t key_bytes = 0;
+    void release();
+};
+
+struct ProfRec {
+    int kind = 0;
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Steps of one partition level, in launch order, and their profiling kinds.
+constexpr int kLevelSteps = 7;
+inline int step_kind(int step) {
+    // splitters, tileplan, count, scan_reduce, scan_apply, plan, scatter
+    static const int kinds[kLevelSteps] = {0, 1, 2, 3, 3, 4, 5};
+    return kinds[step];
+}
+hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st);
+
+Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
+uint32_t levels_needed(uint64_t n, const gqs_params& p);
+
+struct Ctx {
+    int device = 0;
+    int cus = 256;
+    int lds_per_wg = 65536;
+    int wave = 64;
+    std::string name;
+    gqs_params params{};
+    std::mutex mu;
+    std::string err;
+    Workspace ws;
+    hipStream_t last_stream = nullptr;
+    gqs_stats stats{};
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+    // device copies for the host-pointer entry point
+    void* host_d = nullptr;
+    void* host_dn = nullptr;
+    size_t host_cap = 0;
+
+    bool fail(hipError_t e, const char* what);
+    gqs_status set_error(gqs_status s, const char* msg);
+    gqs_status ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
+    Grid grid() const;
+    void prof_begin(int kind, hipStream_t st);
+    void prof_end(hipStream_t st);
+    gqs_status run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets, uint32_t nseg_in,
+                   hipStream_t st);
+    gqs_status fetch_stats();
+};
+
+}  // namespace gqs
+
+struct gqs_ctx : gqs::Ctx {};

@@ -1,0 +1,73 @@
+// kernels.h -- host-side launch interface of the device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "records.h"
+
+namespace gqs {
+
+// Per-bucket result of one partition level: where the bucket starts inside
+// its segment and how many keys it holds.  Bit 31 of `size` marks a bucket
+// whose keys were written straight to their final place in d (an equality
+// bucket, the reference's pivot fill, or a bucket of at most one key).
+struct BucketRec {
+    uint32_t rel;
+    uint32_t size;
+};
+constexpr uint32_t kBucketDirect = 0x80000000u;
+
+// Everything one partition level reads and writes.  Device pointers only.
+struct LevelArgs {
+    void* d;                 // caller keys (in/out)
+    void* dn;                // caller scratch
+    const SegRec* segs;      // this level's segments
+    SegRec* next_segs;       // next level's segments (appended)
+    SegPlan* plans;          // per segment
+    void* spl;               // splitters, kSplStride per segment (K)
+    uint16_t* meta;          // bucket-id table, kSplStride per segment
+    uint32_t* mat;           // count matrix (scanned in place)
+    uint32_t* part;          // scan chunk partials
+    BucketRec* bkt;          // per bucket of this level
+    LevelCtl* ctl;           // this level
+    LevelCtl* ctl_next;      // next level
+    uint32_t mode;           // Mode
+    uint32_t leaf_max;       // buckets up to this size are sorted locally
+    uint32_t leaf_target;    // multipivot: bucket size the fan-out aims for
+    uint32_t oversample;     // multipivot: sample keys per bucket
+    uint32_t max_fanout;     // multipivot: largest fan-out (power of two <= 256)
+    uint32_t is_last;        // 1: no next level, every bucket is sorted locally
+    uint32_t seg_cap;        // capacity of next_segs
+    uint32_t mat_cap;        // capacity of mat
+    uint32_t bkt_cap;        // capacity of bkt
+    uint32_t seed;           // sample-position hash seed
+};
+
+// Local-sort (leaf) arguments.  Either an explicit list of segments
+// (list != nullptr) or every qualifying bucket of one level.
+struct LeafArgs {
+    void* d;
+    void* dn;
+    const SegRec* list;      // explicit leaves (raw or not), count in *list_count
+    const uint32_t* list_count;
+    LevelArgs lv;            // bucket mode: the level whose buckets to sort
+    uint32_t leaf_max;
+    uint32_t is_last;
+};
+
+struct Grid {
+    int splitters;  // work-groups of the splitter kernel
+    int part;       // work-groups of count / scatter
+    int scan;       // work-groups of the scan kernels
+    int plan;       // work-groups of the plan kernel
+    int leaf;       // work-groups of the local sort
+};
+
+// Launchers (all stream-ordered, no host synchronisation).
+hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n,
+                       uint32_t flags, hipStream_t st);
+hipError_t launch_leaf(int dtype, const LeafArgs& a, const Grid& g, hipStream_t st);
+hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, SegRec* segs, LevelCtl* ctl0,
+                                SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max, hipStream_t st);
+
+}  // namespace gqs

@@ -1,0 +1,741 @@
+// kernels.hip -- gfx950 kernels of the GPU-Quicksort hot path.
+//
+// One partition level (the reference's gqsort_kernel launch plus the host
+// bookkeeping around it, CL12/QuicksortMain.cpp:332-373 and
+// CL12/QuicksortKernels.cl:102-240) is a fixed chain of stream-ordered
+// kernels over HBM-resident queues, so no level needs the host:
+//
+//   k_splitters  per segment: pivot(s).  MODE_CLASSIC takes the median of the
+//                first, middle and last key (CL12/QuicksortKernels.cl:218-225);
+//                MODE_MULTIPIVOT sorts a stratified sample in LDS and keeps up
+//                to 255 quantile splitters, giving repeated values their own
+//                equality bucket (the generalisation of the reference's
+//                "== pivot" class, whose keys are filled in place, :214-216).
+//   k_tileplan   prefix sums over segments: tile, count-matrix and bucket bases
+//                (the role of the host's blocksize/parent/block loop, :333-354).
+//   k_count      persistent over tiles: per-tile bucket histogram in LDS,
+//                wave-aggregated through ballot (the reference's per-thread
+//                lt/gt counts, :135-145).
+//   k_scan_*     exclusive scan of the bucket-major count matrix: every tile's
+//                output offset per bucket (replaces the per-parent atomics on
+//                sstart/send, :176-187; deterministic and XCD-safe).
+//   k_plan       per bucket: size, placement flag, child segments appended to
+//                the next level's queue (the host's news -> work/done split,
+//                :362-371, and CL20's relauncher_kernel, CL20/QuicksortKernels.cl:132-222).
+//   k_scatter    persistent over tiles: classify again, rank inside the tile
+//                with wave ballots + LDS atomics, stage the tile in LDS grouped
+//                by bucket and store coalesced runs (the scatter pass, :195-203).
+//                Equality buckets and single keys go straight to d in caller
+//                representation (the pivot fill, :214-216, done in parallel).
+//   k_leaf       persistent over buckets: local sort of every bucket of at most
+//                leaf_max keys in LDS (lqsort_kernel, :267-421), written to d.
+//
+// Keys are sorted as unsigned integers: float/double bits go through an
+// order-preserving bijection on the first load and back on the final store.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace gqs {
+
+// ------------------------------------------------------------ key traits
+template <int DT> struct KeyOf { using K = uint32_t; };
+template <> struct KeyOf<DT_F64> { using K = uint64_t; };
+
+template <int DT>
+__device__ __forceinline__ typename KeyOf<DT>::K to_key(typename KeyOf<DT>::K u) {
+    if constexpr (DT == DT_F32) return (u >> 31) ? ~u : (u | 0x80000000u);
+    else if constexpr (DT == DT_F64) return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    else return u;
+}
+template <int DT>
+__device__ __forceinline__ typename KeyOf<DT>::K from_key(typename KeyOf<DT>::K k) {
+    if constexpr (DT == DT_F32) return (k >> 31) ? (k & 0x7FFFFFFFu) : ~k;
+    else if constexpr (DT == DT_F64) return (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    else return k;
+}
+template <class K>
+__device__ __forceinline__ K key_max() { return ~K(0); }
+
+template <int DT>
+__device__ __forceinline__ typename KeyOf<DT>::K load_key(const typename KeyOf<DT>::K* p, uint64_t i, bool raw) {
+    typename KeyOf<DT>::K v = p[i];
+    return raw ? to_key<DT>(v) : v;
+}
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Exclusive work-group scan of one value per thread (NT threads).
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        uint32_t y = __shfl_up(x, dd, 64);
+        if (lane >= (uint32_t)dd) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        uint32_t t = wsum[i];
+        pre += (i < (int)w) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+template <int NT>
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
+    uint32_t tot;
+    block_excl_scan<NT>(v, wsum, &tot);
+    return tot;
+}
+
+// Add one to hist[b] for every active lane.  A wave whose lanes all carry the
+// same bucket (sorted runs, all-equal input) issues one LDS atomic.
+__device__ __forceinline__ void hist_add(uint32_t* hist, uint32_t b) {
+    const uint64_t act = __ballot(1);
+    const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+    const uint64_t same = __ballot(b == b0);
+    if (same == act) {
+        if (mbcnt(act) == 0) atomicAdd(&hist[b0], (uint32_t)__popcll(act));
+    } else {
+        atomicAdd(&hist[b], 1u);
+    }
+}
+
+// Same, returning each lane's rank among the tile's keys of its bucket.
+__device__ __forceinline__ uint32_t hist_rank(uint32_t* hist, uint32_t b) {
+    const uint64_t act = __ballot(1);
+    const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+    const uint64_t same = __ballot(b == b0);
+    uint32_t r;
+    if (same == act) {
+        const uint32_t below = mbcnt(act);
+        uint32_t base = 0;
+        if (below == 0) base = atomicAdd(&hist[b0], (uint32_t)__popcll(act));
+        base = __builtin_amdgcn_readfirstlane(base);
+        r = base + below;
+    } else {
+        r = atomicAdd(&hist[b], 1u);
+    }
+    return r;
+}
+
+// Splitter classification: i = #{splitters < key} by a branch-free search over
+// 2^depth - 1 sorted entries (padded with the maximum key), then the bucket
+// id from the meta table: base[i] plus one when u[i] has an equality bucket
+// and key == u[i].
+template <class K>
+__device__ __forceinline__ uint32_t classify(K key, const K* sK, const uint16_t* sMeta, uint32_t depth) {
+    uint32_t i = 0;
+    for (uint32_t step = (1u << depth) >> 1; step > 0; step >>= 1) i += (sK[i + step - 1] < key) ? step : 0u;
+    const uint32_t meta = sMeta[i];
+    return (meta & 0xFFu) + (((meta >> 8) & 1u) & (uint32_t)(key == sK[i]));
+}
+
+// Largest s < count with field(s) <= t, for strictly increasing fields.
+template <class F>
+__device__ __forceinline__ uint32_t upper_find(uint32_t count, uint32_t t, F field) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (field(mid) <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t a, uint32_t b, uint32_t c) {
+    uint64_t x = ((uint64_t)a << 32) ^ ((uint64_t)b * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)c << 17) ^ c;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (uint32_t)x;
+}
+
+// In-LDS bitonic sort of buf[0..n) with all comparators ascending: the first
+// step of each merge pairs i with its mirror i ^ (k-1), the later steps pair
+// i with i + h.  Positions >= n act as +infinity, so a pair whose upper index
+// is >= n is left alone and no padding is stored.
+template <class K>
+__device__ void lds_bitonic(K* buf, uint32_t n, uint32_t tid, uint32_t nthreads) {
+    if (n < 2) return;
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    const uint32_t half = P >> 1;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        const uint32_t hk = k >> 1;
+        for (uint32_t q = tid; q < half; q += nthreads) {
+            const uint32_t i = ((q & ~(hk - 1)) << 1) | (q & (hk - 1));
+            const uint32_t j = i ^ (k - 1);
+            if (j < n) {
+                K a = buf[i], b = buf[j];
+                if (b < a) { buf[i] = b; buf[j] = a; }
+            }
+        }
+        __syncthreads();
+        for (uint32_t h = hk >> 1; h >= 1; h >>= 1) {
+            for (uint32_t q = tid; q < half; q += nthreads) {
+                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
+                const uint32_t j = i + h;
+                if (j < n) {
+                    K a = buf[i], b = buf[j];
+                    if (b < a) { buf[i] = b; buf[j] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Same network over a global range (unbounded leaves only; one work-group).
+template <class K>
+__device__ void global_bitonic(K* buf, uint64_t n, uint32_t tid, uint32_t nthreads) {
+    if (n < 2) return;
+    uint64_t P = 1;
+    while (P < n) P <<= 1;
+    const uint64_t half = P >> 1;
+    for (uint64_t k = 2; k <= P; k <<= 1) {
+        for (uint64_t h = k >> 1; h >= 1; h >>= 1) {
+            const bool flip = (h == (k >> 1));
+            for (uint64_t q = tid; q < half; q += nthreads) {
+                const uint64_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
+                const uint64_t j = flip ? (i ^ (k - 1)) : (i + h);
+                if (j < n) {
+                    K a = buf[i], b = buf[j];
+                    if (b < a) { buf[i] = b; buf[j] = a; }
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------ init / seed
+__global__ void k_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n, uint32_t flags) {
+    const uint32_t words = (uint32_t)(nlevels * (int)(sizeof(LevelCtl) / 4));
+    uint32_t* w = reinterpret_cast<uint32_t*>(ctl);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0 && seg0_count) {
+        seg0[0] = SegRec{0, (uint32_t)n, flags};
+        if (flags & 0x100u) ctl[0].nleaf_list = 1;  // the single range is a leaf
+        else ctl[0].nseg = 1;
+        seg0[0].flags &= 0xFFu;
+    }
+}
+
+// Batched sort: one segment or leaf per caller array (offsets[i]..offsets[i+1]).
+__global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, SegRec* segs, LevelCtl* ctl0, SegRec* leaves,
+                       uint32_t* nleaves, uint32_t leaf_max) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg_in; i += gridDim.x * blockDim.x) {
+        const uint64_t lo = offsets[i], hi = offsets[i + 1];
+        const uint64_t len = hi - lo;
+        if (len < 2) continue;
+        SegRec r{lo, (uint32_t)len, SEG_RAW};
+        if (len > leaf_max) segs[atomicAdd(&ctl0->nseg, 1u)] = r;
+        else leaves[atomicAdd(nleaves, 1u)] = r;
+    }
+}
+
+// ------------------------------------------------------------ k_splitters
+template <int DT>
+__global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
+    using K = typename KeyOf<DT>::K;
+    __shared__ K samp[kSampleMax];
+    __shared__ K cand[kSplStride];
+    __shared__ uint32_t wsum[kPartWG / 64];
+    __shared__ K s_pivot;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nseg = a.ctl->nseg;
+    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const SegRec seg = a.segs[s];
+        const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
+        const bool raw = seg.flags & SEG_RAW;
+        const uint32_t n = seg.n;
+        K* spl = (K*)a.spl + (size_t)s * kSplStride;
+        uint16_t* meta = a.meta + (size_t)s * kSplStride;
+        // Candidate splitters cand[0..ncand), sorted, possibly with repeats.
+        uint32_t ncand;
+        if (a.mode == MODE_CLASSIC) {
+            if (tid == 0) {
+                const K x1 = load_key<DT>(src, seg.start, raw);
+                const K x2 = load_key<DT>(src, seg.start + (n >> 1), raw);
+                const K x3 = load_key<DT>(src, seg.start + n - 1, raw);
+                // median-of-3, CL12/Quicksort.h:59-73
+                K m;
+                if (x1 < x2) m = (x2 < x3) ? x2 : ((x1 < x3) ? x3 : x1);
+                else m = (x1 < x3) ? x1 : ((x2 < x3) ? x3 : x2);
+                s_pivot = m;
+                cand[0] = m;
+                cand[1] = m;  // a repeated candidate: the pivot gets its equality bucket
+            }
+            ncand = 2;
+        } else {
+            uint32_t k = 2;
+            const uint32_t want = (n + a.leaf_target - 1) / a.leaf_target;
+            while (k < want && k < a.max_fanout) k <<= 1;
+            uint32_t S = 1;
+            while (S < k * a.oversample && S < (uint32_t)kSampleMax) S <<= 1;
+            while (S > n) S >>= 1;
+            while (k > 2 && k > S / 2) k >>= 1;
+            for (uint32_t j = tid; j < S; j += kPartWG) {
+                const uint64_t r = hash32(a.seed, (uint32_t)seg.start ^ (uint32_t)(seg.start >> 32), j) % n;
+                const uint64_t pos = ((uint64_t)j * n + r) / S;
+                samp[j] = load_key<DT>(src, seg.start + pos, raw);
+            }
+            __syncthreads();
+            lds_bitonic<K>(samp, S, tid, kPartWG);
+            ncand = k - 1;
+            for (uint32_t i = tid; i < ncand; i += kPartWG) cand[i] = samp[((uint64_t)(i + 1) * S) / k];
+        }
+        __syncthreads();
+        // Unique splitters u[] with an equality flag for values repeated among
+        // the candidates (one thread per candidate; ncand <= 255).
+        K c = key_max<K>();
+        uint32_t first = 0, eq = 0;
+        if (tid < ncand) {
+            c = cand[tid];
+            first = (tid == 0 || cand[tid - 1] != c) ? 1u : 0u;
+            eq = (first && tid + 1 < ncand && cand[tid + 1] == c) ? 1u : 0u;
+        }
+        uint32_t m;
+        const uint32_t uidx = block_excl_scan<kPartWG>(first, wsum, &m);
+        uint32_t neq;
+        // eq flag moves with its unique index: scan eq over unique order.
+        const uint32_t eqbefore = block_excl_scan<kPartWG>(eq, wsum, &neq);
+        // Table entry i (0 <= i <= 255): splitter u[i] (or max padding) and
+        // meta = base[i] | eq[i] << 8 with base[i] = i + #eq among u[0..i).
+        // Written by the thread that owns unique index i.
+        if (first) {
+            spl[uidx] = c;
+            meta[uidx] = (uint16_t)((uidx + eqbefore) | (eq << 8));
+        }
+        const uint32_t nb = m + 1 + neq;
+        for (uint32_t i = m + tid; i < (uint32_t)kSplStride; i += kPartWG) {
+            spl[i] = key_max<K>();
+            meta[i] = (uint16_t)((i == m) ? (m + neq) : (nb - 1));
+        }
+        if (tid == 0) {
+            uint32_t depth = 0;
+            while (((1u << depth) - 1u) < m) ++depth;
+            SegPlan p;
+            p.tile_begin = 0;
+            p.ntiles = (n + TileShape<K>::kTileKeys - 1) / TileShape<K>::kTileKeys;
+            p.mat_base = 0;
+            p.nbuckets = nb;
+            p.bucket_base = 0;
+            p.depth = depth;
+            a.plans[s] = p;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ k_tileplan
+constexpr int kPlanWG = 1024;
+__global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
+    __shared__ uint32_t wsum[kPlanWG / 64];
+    const uint32_t nseg = a.ctl->nseg;
+    uint32_t ct = 0, cm = 0, cb = 0;
+    for (uint32_t base = 0; base < nseg; base += kPlanWG) {
+        const uint32_t s = base + threadIdx.x;
+        uint32_t nt = 0, nb = 0;
+        if (s < nseg) {
+            nt = a.plans[s].ntiles;
+            nb = a.plans[s].nbuckets;
+        }
+        uint32_t tt, tm, tb;
+        const uint32_t et = block_excl_scan<kPlanWG>(nt, wsum, &tt);
+        const uint32_t em = block_excl_scan<kPlanWG>(nt * nb, wsum, &tm);
+        const uint32_t eb = block_excl_scan<kPlanWG>(nb, wsum, &tb);
+        if (s < nseg) {
+            a.plans[s].tile_begin = ct + et;
+            a.plans[s].mat_base = cm + em;
+            a.plans[s].bucket_base = cb + eb;
+        }
+        ct += tt;
+        cm += tm;
+        cb += tb;
+    }
+    if (threadIdx.x == 0) {
+        if (cm > a.mat_cap || cb > a.bkt_cap) {
+            // Capacity is sized from provable bounds; should this ever fire the
+            // level is skipped and the host reports GQS_EOVERFLOW.
+            a.ctl->overflow |= 1u;
+            a.ctl->nseg = 0;
+            ct = cm = cb = 0;
+        }
+        a.ctl->total_tiles = ct;
+        a.ctl->mat_size = cm;
+        a.ctl->total_buckets = cb;
+        a.ctl->nchunks = (cm + kScanChunk - 1) / kScanChunk;
+    }
+}
+
+// ------------------------------------------------------------ tile helpers
+template <class K>
+struct TileLds {
+    K sK[kSplStride];
+    uint16_t sMeta[kSplStride];
+    uint32_t hist[kSplStride];
+    uint32_t seg;
+    uint32_t cur;
+};
+
+template <class K>
+__device__ __forceinline__ void load_splitters(const LevelArgs& a, uint32_t s, K* sK, uint16_t* sMeta) {
+    const K* spl = (const K*)a.spl + (size_t)s * kSplStride;
+    const uint16_t* meta = a.meta + (size_t)s * kSplStride;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kSplStride; i += blockDim.x) {
+        sK[i] = spl[i];
+        sMeta[i] = meta[i];
+    }
+}
+
+// ------------------------------------------------------------ k_count
+template <int DT>
+__global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
+    using K = typename KeyOf<DT>::K;
+    using TS = TileShape<K>;
+    __shared__ TileLds<K> L;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t total = a.ctl->total_tiles;
+    const uint32_t nseg = a.ctl->nseg;
+    uint32_t cur = 0xFFFFFFFFu;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        if (tid == 0) L.seg = upper_find(nseg, t, [&](uint32_t i) { return a.plans[i].tile_begin; });
+        __syncthreads();
+        const uint32_t s = L.seg;
+        if (s != cur) {
+            load_splitters<K>(a, s, L.sK, L.sMeta);
+            cur = s;
+        }
+        L.hist[tid] = 0;
+        __syncthreads();
+        const SegRec seg = a.segs[s];
+        const SegPlan p = a.plans[s];
+        const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
+        const bool raw = seg.flags & SEG_RAW;
+        const uint32_t tl = t - p.tile_begin;
+        const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
+        const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
+        const K* base = src + seg.start;
+        for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
+            K keys[TS::kKeysPerThread];
+#pragma unroll
+            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+                const uint32_t idx = sub + j * kPartWG + tid;
+                keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+            }
+#pragma unroll
+            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+                const uint32_t idx = sub + j * kPartWG + tid;
+                if (idx < hi) hist_add(L.hist, classify<K>(keys[j], L.sK, L.sMeta, p.depth));
+            }
+        }
+        __syncthreads();
+        for (uint32_t b = tid; b < p.nbuckets; b += kPartWG) a.mat[p.mat_base + b * p.ntiles + tl] = L.hist[b];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ scan
+__global__ __launch_bounds__(256) void k_scan_reduce(LevelArgs a) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t nch = a.ctl->nchunks, size = a.ctl->mat_size;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint32_t lo = c * kScanChunk;
+        const uint32_t hi = min(lo + (uint32_t)kScanChunk, size);
+        uint32_t v = 0;
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) v += a.mat[i];
+        v = block_sum<256>(v, wsum);
+        if (threadIdx.x == 0) a.part[c] = v;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(LevelArgs a) {
+    __shared__ uint32_t wsum[4];
+    constexpr int PER = kScanChunk / 256;
+    const uint32_t nch = a.ctl->nchunks, size = a.ctl->mat_size;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        uint32_t pre = 0;
+        for (uint32_t i = threadIdx.x; i < c; i += 256) pre += a.part[i];
+        pre = block_sum<256>(pre, wsum);
+        const uint32_t lo = c * kScanChunk + threadIdx.x * PER;
+        uint32_t v[PER];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            v[j] = (lo + j < size) ? a.mat[lo + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint32_t run = pre + block_excl_scan<256>(sum, wsum, &tot);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (lo + j < size) a.mat[lo + j] = run;
+            run += v[j];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ k_plan
+__global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
+    __shared__ uint8_t eqb[kSplStride + 1];
+    const uint32_t nseg = a.ctl->nseg;
+    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const SegRec seg = a.segs[s];
+        const SegPlan p = a.plans[s];
+        for (uint32_t i = threadIdx.x; i <= (uint32_t)kSplStride; i += 256) eqb[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kSplStride; i += 256) {
+            const uint16_t m = a.meta[(size_t)s * kSplStride + i];
+            if (m >> 8) eqb[(m & 0xFFu) + 1] = 1;
+        }
+        __syncthreads();
+        const uint32_t m0 = a.mat[p.mat_base];
+        for (uint32_t b = threadIdx.x; b < p.nbuckets; b += 256) {
+            const uint32_t rel = a.mat[p.mat_base + b * p.ntiles] - m0;
+            const uint32_t rel1 = (b + 1 < p.nbuckets) ? a.mat[p.mat_base + (b + 1) * p.ntiles] - m0 : seg.n;
+            const uint32_t size = rel1 - rel;
+            const bool direct = eqb[b] || size <= 1;
+            a.bkt[p.bucket_base + b] = BucketRec{rel, size | (direct ? kBucketDirect : 0u)};
+            if (!direct && size > a.leaf_max && !a.is_last) {
+                const uint32_t slot = atomicAdd(&a.ctl_next->nseg, 1u);
+                if (slot < a.seg_cap) {
+                    a.next_segs[slot] = SegRec{seg.start + rel, size, (seg.flags & SEG_SRC_DN) ^ SEG_SRC_DN};
+                } else {
+                    atomicOr(&a.ctl_next->overflow, 2u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ k_scatter
+template <int DT>
+__global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
+    using K = typename KeyOf<DT>::K;
+    using TS = TileShape<K>;
+    __shared__ TileLds<K> L;
+    __shared__ uint32_t lstart[kSplStride];
+    __shared__ uint32_t cursor[kSplStride];
+    __shared__ uint8_t direct[kSplStride];
+    __shared__ K stage[TS::kSubKeys];
+    __shared__ uint8_t sbk[TS::kSubKeys];
+    __shared__ uint32_t wsum[kPartWG / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t total = a.ctl->total_tiles;
+    const uint32_t nseg = a.ctl->nseg;
+    uint32_t cur = 0xFFFFFFFFu;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        if (tid == 0) L.seg = upper_find(nseg, t, [&](uint32_t i) { return a.plans[i].tile_begin; });
+        __syncthreads();
+        const uint32_t s = L.seg;
+        if (s != cur) {
+            load_splitters<K>(a, s, L.sK, L.sMeta);
+            cur = s;
+        }
+        const SegRec seg = a.segs[s];
+        const SegPlan p = a.plans[s];
+        const uint32_t tl = t - p.tile_begin;
+        if (tid < p.nbuckets) {
+            const uint32_t m0 = a.mat[p.mat_base];
+            cursor[tid] = a.mat[p.mat_base + tid * p.ntiles + tl] - m0;
+            direct[tid] = (a.bkt[p.bucket_base + tid].size & kBucketDirect) ? 1 : 0;
+        }
+        const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
+        K* dst = (seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn;
+        K* dfinal = (K*)a.d;
+        const bool raw = seg.flags & SEG_RAW;
+        const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
+        const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
+        const K* base = src + seg.start;
+        for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
+            L.hist[tid] = 0;
+            __syncthreads();
+            K keys[TS::kKeysPerThread];
+            uint32_t bk[TS::kKeysPerThread];
+            uint32_t rk[TS::kKeysPerThread];
+#pragma unroll
+            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+                const uint32_t idx = sub + j * kPartWG + tid;
+                keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+            }
+#pragma unroll
+            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+                const uint32_t idx = sub + j * kPartWG + tid;
+                bk[j] = 0;
+                rk[j] = 0;
+                if (idx < hi) {
+                    bk[j] = classify<K>(keys[j], L.sK, L.sMeta, p.depth);
+                    rk[j] = hist_rank(L.hist, bk[j]);
+                }
+            }
+            __syncthreads();
+            uint32_t cnt;
+            const uint32_t hv = (tid < p.nbuckets) ? L.hist[tid] : 0u;
+            const uint32_t ls = block_excl_scan<kPartWG>(hv, wsum, &cnt);
+            lstart[tid] = ls;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+                const uint32_t idx = sub + j * kPartWG + tid;
+                if (idx < hi) {
+                    const uint32_t pos = lstart[bk[j]] + rk[j];
+                    stage[pos] = keys[j];
+                    sbk[pos] = (uint8_t)bk[j];
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < cnt; i += kPartWG) {
+                const uint32_t b = sbk[i];
+                const uint64_t pos = seg.start + cursor[b] + (i - lstart[b]);
+                const K key = stage[i];
+                if (direct[b]) dfinal[pos] = from_key<DT>(key);
+                else dst[pos] = key;
+            }
+            __syncthreads();
+            if (tid < p.nbuckets) cursor[tid] += hv;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ k_leaf
+// One work-group per leaf; LDS holds the leaf (dynamic size leaf_max keys).
+template <int DT>
+__global__ __launch_bounds__(kLeafWG) void k_leaf(LeafArgs a) {
+    using K = typename KeyOf<DT>::K;
+    extern __shared__ unsigned char leaf_smem[];
+    K* buf = reinterpret_cast<K*>(leaf_smem);
+    __shared__ uint32_t s_seg;
+    const uint32_t tid = threadIdx.x;
+    const bool list_mode = a.list != nullptr;
+    const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
+    const uint32_t nseg = list_mode ? 0u : a.lv.ctl->nseg;
+    for (uint32_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        uint64_t start;
+        uint32_t n;
+        bool from_dn, raw;
+        if (list_mode) {
+            const SegRec r = a.list[idx];
+            start = r.start;
+            n = r.n;
+            from_dn = r.flags & SEG_SRC_DN;
+            raw = r.flags & SEG_RAW;
+        } else {
+            if (tid == 0) s_seg = upper_find(nseg, idx, [&](uint32_t i) { return a.lv.plans[i].bucket_base; });
+            __syncthreads();
+            const uint32_t s = s_seg;
+            __syncthreads();
+            const BucketRec br = a.lv.bkt[idx];
+            if (br.size & kBucketDirect) continue;
+            n = br.size;
+            if (n > a.leaf_max && !a.is_last) continue;  // partitioned again at the next level
+            const SegRec seg = a.lv.segs[s];
+            start = seg.start + br.rel;
+            from_dn = !(seg.flags & SEG_SRC_DN);
+            raw = false;
+        }
+        if (n == 0) continue;
+        const K* src = from_dn ? (const K*)a.dn : (const K*)a.d;
+        K* dout = (K*)a.d;
+        if (n <= a.leaf_max) {
+            for (uint32_t i = tid; i < n; i += kLeafWG) buf[i] = load_key<DT>(src, start + i, raw);
+            __syncthreads();
+            lds_bitonic<K>(buf, n, tid, kLeafWG);
+            for (uint32_t i = tid; i < n; i += kLeafWG) dout[start + i] = from_key<DT>(buf[i]);
+            __syncthreads();
+        } else {
+            // Unbounded fallback: sort in place in the source buffer.
+            K* g = (K*)src + start;
+            if (raw)
+                for (uint32_t i = tid; i < n; i += kLeafWG) g[i] = to_key<DT>(g[i]);
+            __threadfence_block();
+            __syncthreads();
+            global_bitonic<K>(g, n, tid, kLeafWG);
+            for (uint32_t i = tid; i < n; i += kLeafWG) dout[start + i] = from_key<DT>(g[i]);
+            if (tid == 0) atomicAdd(&a.lv.ctl->oversize, 1u);
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n, uint32_t flags,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, st, ctl, nlevels, seg0, seg0_count, n, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, SegRec* segs, LevelCtl* ctl0,
+                                SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max, hipStream_t st) {
+    const uint32_t blocks = std::max(1u, std::min(1024u, (nseg_in + 255) / 256));
+    hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(256), 0, st, offsets, nseg_in, segs, ctl0, leaves, nleaves,
+                       leaf_max);
+    return hipGetLastError();
+}
+
+template <int DT>
+static hipError_t level_step_t(int step, const LevelArgs& a, const Grid& g, hipStream_t st) {
+    switch (step) {
+        case 0: hipLaunchKernelGGL(k_splitters<DT>, dim3(g.splitters), dim3(kPartWG), 0, st, a); break;
+        case 1: hipLaunchKernelGGL(k_tileplan, dim3(1), dim3(kPlanWG), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(k_count<DT>, dim3(g.part), dim3(kPartWG), 0, st, a); break;
+        case 3: hipLaunchKernelGGL(k_scan_reduce, dim3(g.scan), dim3(256), 0, st, a); break;
+        case 4: hipLaunchKernelGGL(k_scan_apply, dim3(g.scan), dim3(256), 0, st, a); break;
+        case 5: hipLaunchKernelGGL(k_plan, dim3(g.plan), dim3(256), 0, st, a); break;
+        case 6: hipLaunchKernelGGL(k_scatter<DT>, dim3(g.part), dim3(kPartWG), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st) {
+    switch (dtype) {
+        case DT_U32: return level_step_t<DT_U32>(step, a, g, st);
+        case DT_F32: return level_step_t<DT_F32>(step, a, g, st);
+        case DT_F64: return level_step_t<DT_F64>(step, a, g, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int DT>
+static hipError_t leaf_t(const LeafArgs& a, const Grid& g, hipStream_t st) {
+    using K = typename KeyOf<DT>::K;
+    const size_t smem = (size_t)a.leaf_max * sizeof(K);
+    hipLaunchKernelGGL(k_leaf<DT>, dim3(g.leaf), dim3(kLeafWG), smem, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf(int dtype, const LeafArgs& a, const Grid& g, hipStream_t st) {
+    switch (dtype) {
+        case DT_U32: return leaf_t<DT_U32>(a, g, st);
+        case DT_F32: return leaf_t<DT_F32>(a, g, st);
+        case DT_F64: return leaf_t<DT_F64>(a, g, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gqs

@@ -1,0 +1,81 @@
+// records.h -- records and tunables shared by the host dispatcher and the
+// device kernels of the GPU-Quicksort hot path.
+//
+// The reference keeps three records per level (CL12/Quicksort.h:100-145):
+// work_record {start, end, pivot, direction}, parent_record {sstart, send,
+// oldstart, oldend, blockcount} and block_record {start, end, pivot,
+// direction, parent}.  Here one SegRec plays the work_record role (a key range
+// still to be partitioned, with the buffer it lives in), SegPlan carries what
+// the parent record's counters carried (where each tile's keys go), and the
+// block records are implicit: tile t of a level maps to a segment by a binary
+// search over SegPlan::tile_begin.  Pivots live in a per-segment splitter table
+// instead of in each record so that one segment can have up to 255 splitters.
+#pragma once
+#include <stdint.h>
+
+namespace gqs {
+
+// Key representation: every dtype is sorted as an unsigned integer key.
+enum DType : int { DT_U32 = 0, DT_F32 = 1, DT_F64 = 2 };
+
+// Partition strategy.
+//  MODE_MULTIPIVOT: splitters are quantiles of a sorted sample (up to 255 per
+//                   segment, fan-out sized so buckets land near leaf_target).
+//  MODE_CLASSIC:    one pivot per segment, the median of the first, middle and
+//                   last key (CL12/QuicksortKernels.cl:218-225), leaf cutoff
+//                   QUICKSORT_BLOCK_SIZE: the reference's own schedule.
+enum Mode : int { MODE_MULTIPIVOT = 0, MODE_CLASSIC = 1 };
+
+// SegRec::flags
+constexpr uint32_t SEG_SRC_DN = 1u;  // keys live in dn (else in d)
+constexpr uint32_t SEG_RAW = 2u;     // keys are still in caller representation
+
+// A key range [start, start+n) to partition (or to sort locally).
+struct SegRec {
+    uint64_t start;
+    uint32_t n;
+    uint32_t flags;
+};
+
+// Per-segment plan of one level, written by the splitter and tile-plan kernels.
+struct SegPlan {
+    uint32_t tile_begin;   // first tile of this segment in the level's tile space
+    uint32_t ntiles;       // ceil(n / tile_keys)
+    uint32_t mat_base;     // first entry of this segment in the count matrix
+    uint32_t nbuckets;     // buckets of this segment (m + 1 + #equality buckets)
+    uint32_t bucket_base;  // first bucket of this segment in the level's bucket space
+    uint32_t depth;        // binary-search depth: 2^depth - 1 >= number of splitters
+};
+
+// Per-level control block in HBM.
+struct LevelCtl {
+    uint32_t nseg;           // segments to partition at this level (appended by the previous level)
+    uint32_t total_tiles;    // sum of SegPlan::ntiles
+    uint32_t mat_size;       // count-matrix entries
+    uint32_t total_buckets;  // sum of SegPlan::nbuckets
+    uint32_t nchunks;        // scan chunks over the count matrix
+    uint32_t nleaf_list;     // explicit leaf records (level 0 only)
+    uint32_t oversize;       // leaves sorted by the unbounded fallback path
+    uint32_t overflow;       // capacity overflow flags (should stay 0)
+};
+
+constexpr int kMaxSplitters = 255;
+constexpr int kSplStride = 256;  // splitter-table stride per segment
+constexpr int kMaxLevels = 48;   // level-control blocks allocated per context
+
+// Partition work-group shape.
+constexpr int kPartWG = 256;
+// Keys per sub-tile: 16 KiB of keys; a tile is kSubPerTile sub-tiles.
+template <class K>
+struct TileShape {
+    static constexpr int kKeysPerThread = 16 * 4 / (int)sizeof(K);  // 16 (u32) / 8 (u64)
+    static constexpr int kSubKeys = kKeysPerThread * kPartWG;      // 4096 / 2048
+    static constexpr int kSubPerTile = 4;
+    static constexpr int kTileKeys = kSubKeys * kSubPerTile;  // 16384 / 8192
+};
+
+constexpr int kScanChunk = 4096;   // count-matrix entries per scan chunk
+constexpr int kSampleMax = 4096;   // largest splitter sample
+constexpr int kLeafWG = 256;       // local-sort work-group size
+
+}  // namespace gqs

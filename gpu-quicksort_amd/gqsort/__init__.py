@@ -1,0 +1,234 @@
+"""Python binding of libgqsort.so (the MI355X GPU-Quicksort C ABI).
+
+The binding mirrors the reference's entry point
+
+    template <class T> void GPUQSort(OCLResources*, size_t size, T* d, T* dn)
+    (GPU-Quicksort_OpenCL_1.2/QuicksortMain.cpp:297-298)
+
+as ``GPUQSort(ctx, size, d, dn)``: ``d`` (numpy, host) holds the keys and is
+sorted in place, ``dn`` is scratch of the same size.  Unlike the reference it
+raises ``GQSError`` on failure instead of asserting.
+
+Device-resident sorts take raw device pointers (or torch tensors) and a
+stream.  There is no CPU fallback: if the shared library is missing or no GPU
+is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgqsort.so")
+
+U32, F32, F64 = 0, 1, 2
+MODE_MULTIPIVOT, MODE_CLASSIC = 0, 1
+DTYPE_CODES = {np.dtype(np.uint32): U32, np.dtype(np.float32): F32, np.dtype(np.float64): F64}
+NP_DTYPES = {U32: np.uint32, F32: np.float32, F64: np.float64}
+KEY_BYTES = {U32: 4, F32: 4, F64: 8}
+
+STATUS = {0: "GQS_OK", 1: "GQS_EINVAL", 2: "GQS_ENOMEM", 3: "GQS_EDEVICE", 4: "GQS_EOVERFLOW",
+          5: "GQS_EBUSY", 6: "GQS_ENOTFOUND"}
+
+# Generators of gqs_generate.
+GEN_UNIFORM, GEN_GAUSSIAN, GEN_SORTED, GEN_REVERSE, GEN_EQUAL, GEN_BUCKET, GEN_PERMUTATION, GEN_NORMAL_FP = range(8)
+GENERATORS = {"uniform": 0, "gaussian": 1, "sorted": 2, "reverse": 3, "equal": 4, "bucket": 5,
+              "permutation": 6, "normal": 7}
+
+KERNEL_KINDS = ["splitters", "tileplan", "count", "scan", "plan", "scatter", "leaf", "init"]
+
+
+class GQSError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("leaf_max", ctypes.c_uint32), ("leaf_target", ctypes.c_uint32),
+                ("oversample", ctypes.c_uint32), ("max_fanout", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
+                ("host_driven", ctypes.c_uint32), ("part_wgs_per_cu", ctypes.c_uint32),
+                ("leaf_wgs_per_cu", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("levels_launched", ctypes.c_uint32), ("levels_used", ctypes.c_uint32),
+                ("oversize_leaves", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
+                ("segments", ctypes.c_uint32 * 8)]
+
+
+# Every symbol include/gqsort.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_SIGS = {
+    "gqs_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
+    "gqs_destroy": (ctypes.c_int, [_P]),
+    "gqs_last_error": (ctypes.c_char_p, [_P]),
+    "gqs_get_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
+    "gqs_set_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
+    "gqs_default_params": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params)]),
+    "gqs_sort": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
+    "gqs_sort_device": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
+    "gqs_sort_segmented": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t, _P]),
+    "gqs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "gqs_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gqs_kernel_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.c_int]),
+    "gqs_device_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_size_t]),
+    "gqs_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _P, ctypes.c_size_t]),
+}
+OPTIONAL_SIGS = {}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libgqsort.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run gpu-quicksort_amd/build.py")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in {**_SIGS, **OPTIONAL_SIGS}.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor, or an int address."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return int(x)
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if hasattr(s, "cuda_stream"):
+        return s.cuda_stream
+    return int(s)
+
+
+def default_params(dtype=U32):
+    p = Params()
+    _check(load().gqs_default_params(dtype, ctypes.byref(p)))
+    return p
+
+
+def _check(st, ctx=None):
+    if st != 0:
+        msg = ""
+        if ctx is not None and ctx._h:
+            m = load().gqs_last_error(ctx._h)
+            msg = m.decode() if m else ""
+        raise GQSError(st, msg)
+
+
+class Context:
+    """Owns a gqs_ctx: device, workspace, tuned parameters."""
+
+    def __init__(self, device=0):
+        self._lib = load()
+        h = _P()
+        _check(self._lib.gqs_create(ctypes.byref(h), device))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.gqs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self):
+        p = Params()
+        _check(self._lib.gqs_get_params(self._h, ctypes.byref(p)), self)
+        return p
+
+    def set_params(self, p=None, **kw):
+        q = self.params() if p is None else p
+        for k, v in kw.items():
+            setattr(q, k, v)
+        _check(self._lib.gqs_set_params(self._h, ctypes.byref(q)), self)
+        return q
+
+    def sort_host(self, d, dn=None):
+        """Reference semantics: host arrays, synchronous, d sorted in place."""
+        d = np.ascontiguousarray(d) if not d.flags.c_contiguous else d
+        dt = DTYPE_CODES[d.dtype]
+        if dn is None:
+            dn = np.empty_like(d)
+        _check(self._lib.gqs_sort(self._h, dt, d.ctypes.data, dn.ctypes.data, d.size), self)
+        return d
+
+    def sort_device(self, dtype, d, dn, n, stream=None):
+        _check(self._lib.gqs_sort_device(self._h, dtype, _ptr(d), _ptr(dn), n, _stream(stream)), self)
+
+    def sort_segmented(self, dtype, d, dn, offsets, nseg, total, stream=None):
+        _check(self._lib.gqs_sort_segmented(self._h, dtype, _ptr(d), _ptr(dn), _ptr(offsets), nseg, total,
+                                            _stream(stream)), self)
+
+    def stats(self):
+        s = Stats()
+        _check(self._lib.gqs_get_stats(self._h, ctypes.byref(s)), self)
+        return s
+
+    def set_profiling(self, on=True):
+        _check(self._lib.gqs_set_profiling(self._h, 1 if on else 0), self)
+
+    def kernel_times(self):
+        n = len(KERNEL_KINDS)
+        launches = (ctypes.c_uint32 * n)()
+        ms = (ctypes.c_double * n)()
+        _check(self._lib.gqs_kernel_times(self._h, launches, ms, n), self)
+        return {k: (launches[i], ms[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+    def device_info(self):
+        cus, lds, wave = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        name = ctypes.create_string_buffer(128)
+        _check(self._lib.gqs_device_info(self._h, ctypes.byref(cus), ctypes.byref(lds), ctypes.byref(wave),
+                                         name, 128), self)
+        return {"cus": cus.value, "lds_per_wg": lds.value, "wave": wave.value, "arch": name.value.decode()}
+
+
+def GPUQSort(ctx, size, d, dn):
+    """Drop-in for GPUQSort<T>(pOCL, size, d, dn): sorts host array d[0:size]
+    ascending in place, clobbering dn (CL12/QuicksortMain.cpp:297-384)."""
+    if d.dtype not in DTYPE_CODES:
+        raise GQSError(1, f"unsupported key type {d.dtype}")
+    if d.size < size or dn.size < size or dn.dtype != d.dtype:
+        raise GQSError(1, "d and dn must hold size keys of the same type")
+    dt = DTYPE_CODES[d.dtype]
+    _check(load().gqs_sort(ctx.handle, dt, d.ctypes.data, dn.ctypes.data, size), ctx)
+
+
+def generate(dtype, gen, seed, n):
+    """Synthetic input of one of the benchmark generators (host numpy array)."""
+    if isinstance(gen, str):
+        gen = GENERATORS[gen]
+    a = np.empty(n, dtype=NP_DTYPES[dtype])
+    _check(load().gqs_generate(dtype, gen, seed, a.ctypes.data if n else None, n))
+    return a

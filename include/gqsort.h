@@ -1,0 +1,125 @@
+/* gqsort.h -- C ABI of the MI355X GPU-Quicksort library (libgqsort.so).
+ *
+ * Drop-in for the reference's sort entry point
+ *     template <class T> void GPUQSort(OCLResources*, size_t size, T* d, T* dn)
+ *     (GPU-Quicksort_OpenCL_1.2/QuicksortMain.cpp:297-298,
+ *      GPU-Quicksort_SYCL/QuicksortMain.cpp:357-358,
+ *      GPU-Quicksort_Full_SYCL/QuicksortMain.cpp:835-836)
+ * and the OpenCL 2.0 device-scheduled variant
+ *     GPUQSort(OCLResources*, size_t size, T* d, work_record* pdone,
+ *              work_record* pnews, size_t MAX_SIZE)
+ *     (GPU-Quicksort_OpenCL_2.0/QuicksortMain.cpp:301-302).
+ *
+ * Contract kept from the reference: `d` holds the keys and receives them
+ * sorted ascending, `dn` is caller-owned scratch of at least n keys that is
+ * clobbered, the library never frees caller memory.  Differences: every call
+ * returns a status instead of asserting (CL12/OpenCLUtils.h:41-45), and keys
+ * may be uint32, float or double (the reference's OpenCL variants sort uint
+ * only; the full-SYCL variant instantiates uint/float/double,
+ * GPU-Quicksort_Full_SYCL/QuicksortMain.cpp:1192-1194).  Float keys are
+ * ordered as IEEE values for inputs without NaN; -0.0 sorts before +0.0.
+ *
+ * Threading: a context serialises its own calls with an internal lock; use one
+ * context per host thread for concurrency.  Streams are passed as void*
+ * (a hipStream_t; NULL is the default stream).
+ */
+#ifndef GQSORT_H
+#define GQSORT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gqs_status {
+    GQS_OK = 0,
+    GQS_EINVAL = 1,      /* null pointer, bad dtype, n out of range, bad parameter */
+    GQS_ENOMEM = 2,      /* device or pinned-host allocation failed */
+    GQS_EDEVICE = 3,     /* a HIP call failed (see gqs_last_error) */
+    GQS_EOVERFLOW = 4,   /* an internal queue bound was exceeded (should never happen) */
+    GQS_EBUSY = 5,       /* serving: memory budget or slots exhausted, request not admitted */
+    GQS_ENOTFOUND = 6    /* serving: unknown ticket */
+} gqs_status;
+
+typedef enum gqs_dtype { GQS_U32 = 0, GQS_F32 = 1, GQS_F64 = 2 } gqs_dtype;
+
+typedef enum gqs_mode {
+    GQS_MODE_MULTIPIVOT = 0, /* sample-chosen splitters, up to 255 per segment (default) */
+    GQS_MODE_CLASSIC = 1     /* one median-of-3 pivot per segment, QBS leaf cutoff (the reference's schedule) */
+} gqs_mode;
+
+typedef struct gqs_ctx gqs_ctx;
+
+/* Tunables (the reference's compile-time QUICKSORT_BLOCK_SIZE,
+ * SORT_THRESHOLD and work-group sizes, CL12/Quicksort.h:77-98, become
+ * run-time parameters chosen by the autotuner). */
+typedef struct gqs_params {
+    int32_t mode;          /* gqs_mode */
+    uint32_t leaf_max;     /* largest bucket sorted in LDS (QUICKSORT_BLOCK_SIZE role); 16..16384 (u32/f32), ..8192 (f64) */
+    uint32_t leaf_target;  /* multipivot: bucket size the fan-out aims for */
+    uint32_t oversample;   /* multipivot: sample keys per bucket */
+    uint32_t max_fanout;   /* multipivot: largest fan-out, power of two in [2, 256] */
+    uint32_t max_levels;   /* partition levels before every bucket goes to the local sort */
+    uint32_t host_driven;  /* 1: read the level queue back after each level and stop early (CL12 loop) */
+    uint32_t part_wgs_per_cu; /* persistent count/scatter work-groups per CU */
+    uint32_t leaf_wgs_per_cu; /* persistent local-sort work-groups per CU */
+} gqs_params;
+
+/* Counters of the last sort on a context. */
+typedef struct gqs_stats {
+    uint32_t levels_launched;   /* partition levels enqueued */
+    uint32_t levels_used;       /* levels that had segments (host_driven or after gqs_sync_stats) */
+    uint32_t oversize_leaves;   /* leaves sorted by the unbounded fallback */
+    uint32_t overflow;          /* non-zero if a queue bound was hit */
+    uint32_t segments[8];       /* segments per level (first 8 levels) */
+} gqs_stats;
+
+/* Context: owns the device, its workspace (grown on demand) and parameters. */
+gqs_status gqs_create(gqs_ctx** ctx, int device);
+gqs_status gqs_destroy(gqs_ctx* ctx);
+const char* gqs_last_error(const gqs_ctx* ctx);
+gqs_status gqs_get_params(const gqs_ctx* ctx, gqs_params* out);
+gqs_status gqs_set_params(gqs_ctx* ctx, const gqs_params* p);
+gqs_status gqs_default_params(int dtype, gqs_params* out);
+
+/* Reference semantics: d, dn are HOST pointers; synchronous.  The keys are
+ * copied to device buffers owned by the context, sorted and copied back. */
+gqs_status gqs_sort(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n);
+
+/* Device-resident: d, dn are device pointers; stream-ordered, returns after
+ * enqueueing (no host synchronisation unless params.host_driven). */
+gqs_status gqs_sort_device(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n, void* stream);
+
+/* Segmented: nseg independent arrays packed in d, array i occupying
+ * [offsets[i], offsets[i+1]) (offsets: nseg+1 device uint64).  Each array is
+ * sorted on its own; one chain of launches for all of them. */
+gqs_status gqs_sort_segmented(gqs_ctx* ctx, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
+                              size_t total_keys, void* stream);
+
+/* Statistics of the last sort (synchronises the context's last stream). */
+gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
+
+/* Per-kernel timing with HIP events on the sort's own stream.  While enabled,
+ * every launch is bracketed by events; gqs_kernel_times() synchronises and
+ * returns, per kernel kind, the number of launches and total milliseconds.
+ * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init. */
+gqs_status gqs_set_profiling(gqs_ctx* ctx, int on);
+gqs_status gqs_kernel_times(gqs_ctx* ctx, uint32_t* launches, double* ms, int nkinds);
+
+/* Device properties used for launch shapes (CUs, LDS per work-group, name). */
+gqs_status gqs_device_info(gqs_ctx* ctx, int* cus, int* lds_per_wg, int* wave_size, char* name, size_t name_len);
+
+/* Synthetic inputs for the benchmark configurations, generated on the host
+ * with a counter-based hash (deterministic for a seed, independent of thread
+ * count).  gen: 0 uniform, 1 gaussian (mean of 4 uniforms), 2 sorted,
+ * 3 reverse sorted, 4 all equal, 5 bucket (256 shuffled runs), 6 permutation of
+ * 1..n (the reference's own data, CL12/QuicksortMain.cpp:415-416),
+ * 7 normal floats with 1% wide outliers and 0.1% duplicates. */
+gqs_status gqs_generate(int dtype, int gen, uint64_t seed, void* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GQSORT_H */

@@ -1,0 +1,97 @@
+"""GPU parity: the HIP sort through the C ABI, bit-exact against std::sort.
+
+The reference checks every GPU result against std::sort of the same input
+(CL12/QuicksortMain.cpp:498-516); these tests do the same on seeded inputs,
+plus the edge sizes around the local-sort and tile cutoffs.
+"""
+import numpy as np
+import pytest
+
+import gqsort
+import orc
+from conftest import gpu_sort
+
+pytestmark = pytest.mark.gpu
+
+GENS = ["uniform", "gaussian", "sorted", "reverse", "equal", "bucket", "permutation"]
+
+
+def check(got, inp, case):
+    exp = orc.std_sort(inp)
+    bad, first = orc.mismatch(got, exp)
+    assert bad == 0, f"{case}: {bad} of {inp.size} keys differ from std::sort, first at {first}"
+
+
+def test_c1_uniform_2p20_seed1(gpu_ctx):
+    """Plain method, small: 2^20 uniform uint32 keys from seed 1."""
+    a = gqsort.generate(gqsort.U32, "uniform", 1, 1 << 20)
+    print("case c1 2^20 uniform seed 1", flush=True)
+    check(gpu_sort(gpu_ctx, a), a, "c1")
+    s = gpu_ctx.stats()
+    assert s.overflow == 0 and s.oversize_leaves == 0
+
+
+@pytest.mark.parametrize("gen", GENS)
+@pytest.mark.parametrize("n", [2, 3, 17, 255, 4097, 8192, 8193, 16385, 65537, 300001])
+def test_sizes_and_generators(gpu_ctx, gen, n):
+    a = gqsort.generate(gqsort.U32, gen, 7, n)
+    print(f"case u32 {gen} n={n}", flush=True)
+    check(gpu_sort(gpu_ctx, a), a, f"{gen}/{n}")
+
+
+@pytest.mark.parametrize("dtype,n", [(gqsort.F32, 1 << 20), (gqsort.F64, 1 << 19), (gqsort.F32, 4099),
+                                     (gqsort.F64, 70001)])
+def test_float_keys(gpu_ctx, dtype, n):
+    a = gqsort.generate(dtype, "normal", 3, n)
+    print(f"case fp dtype={dtype} n={n}", flush=True)
+    check(gpu_sort(gpu_ctx, a), a, f"fp{dtype}/{n}")
+
+
+def test_float_special_values(gpu_ctx):
+    vals = np.array([np.inf, -np.inf, 0.0, 1.0, -1.0, np.finfo(np.float32).max, -np.finfo(np.float32).max,
+                     np.finfo(np.float32).tiny, -np.finfo(np.float32).tiny, 1e-45, -1e-45], dtype=np.float32)
+    rng = np.random.default_rng(5)
+    a = rng.choice(vals, size=20000).astype(np.float32)
+    check(gpu_sort(gpu_ctx, a), a, "f32 specials")
+
+
+@pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
+def test_classic_mode(gpu_ctx, gen):
+    """One median-of-3 pivot per segment and a 1728-key leaf cutoff: the
+    reference's schedule (CL12/Quicksort.h:91-97) on the same kernels."""
+    a = gqsort.generate(gqsort.U32, gen, 11, 1 << 18)
+    print(f"case classic {gen}", flush=True)
+    got = gpu_sort(gpu_ctx, a, mode=gqsort.MODE_CLASSIC, leaf_max=1728, max_levels=40, host_driven=1)
+    check(got, a, f"classic/{gen}")
+    gpu_ctx.set_params(gqsort.default_params())
+
+
+def test_unbounded_fallback(gpu_ctx):
+    """With one level allowed, buckets above leaf_max take the unbounded local sort."""
+    a = gqsort.generate(gqsort.U32, "uniform", 13, 100000)
+    got = gpu_sort(gpu_ctx, a, max_levels=1, leaf_max=64, leaf_target=32, max_fanout=2)
+    check(got, a, "fallback")
+    s = gpu_ctx.stats()
+    assert s.oversize_leaves > 0
+    gpu_ctx.set_params(gqsort.default_params())
+
+
+def test_host_pointer_entry(gpu_ctx):
+    """gqs_sort with host pointers: the reference's GPUQSort(ctx, N, d, dn)."""
+    n = 1 << 18
+    d = gqsort.generate(gqsort.U32, "permutation", 1, n)
+    inp = d.copy()
+    dn = np.empty_like(d)
+    gqsort.GPUQSort(gpu_ctx, n, d, dn)
+    check(d, inp, "GPUQSort host")
+
+
+@pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
+def test_large_2p24_properties(gpu_ctx, gen):
+    import torch
+
+    n = 1 << 24
+    a = gqsort.generate(gqsort.U32, gen, 21, n)
+    print(f"case 2^24 {gen}", flush=True)
+    got = gpu_sort(gpu_ctx, a)
+    check(got, a, f"2^24/{gen}")
