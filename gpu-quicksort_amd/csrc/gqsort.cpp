@@ -61,6 +61,7 @@ void Workspace::release() {
     hipFree(mat);
     hipFree(part);
     hipFree(bkt);
+    hipFree(fill);
     hipFree(ctl);
     hipFree(leaves);
     *this = Workspace{};
@@ -81,6 +82,8 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     c.mat = 256 * (n / tile + 1) + sum_nb + 256;
     c.part = c.mat / kScanChunk + 2;
     c.leaves = std::max<uint64_t>(1, nseg_in);
+    // fill tasks per level: ceil(size/chunk) per equality bucket
+    c.fill = n / kFillChunk + c.bkt + 1;
     return c;
 }
 
@@ -88,7 +91,7 @@ gqs_status Ctx::ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params&
     const Caps need = caps_for(n, nseg_in, p, dtype);
     const size_t kb = key_bytes(dtype);
     if (ws.ctl && need.seg <= ws.cap.seg && need.mat <= ws.cap.mat && need.bkt <= ws.cap.bkt &&
-        need.part <= ws.cap.part && need.leaves <= ws.cap.leaves && kb <= ws.key_bytes)
+        need.part <= ws.cap.part && need.leaves <= ws.cap.leaves && need.fill <= ws.cap.fill && kb <= ws.key_bytes)
         return GQS_OK;
     if (ws.ctl) {
         hipStreamSynchronize(last_stream);
@@ -100,6 +103,7 @@ gqs_status Ctx::ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params&
     c.mat += c.mat / 4;
     c.bkt += c.bkt / 4;
     c.part += c.part / 4 + 1;
+    c.fill += c.fill / 4;
     hipError_t e = hipSuccess;
     if (!e) e = dalloc(&ws.segs[0], c.seg);
     if (!e) e = dalloc(&ws.segs[1], c.seg);
@@ -109,6 +113,7 @@ gqs_status Ctx::ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params&
     if (!e) e = dalloc(&ws.mat, c.mat);
     if (!e) e = dalloc(&ws.part, c.part);
     if (!e) e = dalloc(&ws.bkt, c.bkt);
+    if (!e) e = dalloc(&ws.fill, c.fill);
     if (!e) e = dalloc(&ws.ctl, (size_t)kMaxLevels + 1);
     if (!e) e = dalloc(&ws.leaves, c.leaves);
     if (e) {
@@ -203,6 +208,8 @@ gqs_status Ctx::run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* of
     a.mat = ws.mat;
     a.part = ws.part;
     a.bkt = ws.bkt;
+    a.fill = ws.fill;
+    a.fill_cap = (uint32_t)ws.cap.fill;
     a.mode = (uint32_t)p.mode;
     a.leaf_max = p.leaf_max;
     a.leaf_target = p.leaf_target;
@@ -277,7 +284,10 @@ gqs_status Ctx::fetch_stats() {
         if (c[i].nseg) used = i + 1;
         over += c[i].oversize;
         ovf |= c[i].overflow;
-        if (i < 8) stats.segments[i] = c[i].nseg;
+        if (i < 8) {
+            stats.segments[i] = c[i].nseg;
+            stats.level_keys[i] = ((uint64_t)c[i].keys_hi << 32) | c[i].keys_lo;
+        }
     }
     stats.levels_used = std::max(stats.levels_used, used);
     stats.oversize_leaves = over;

@@ -13,7 +13,7 @@
 namespace gqs {
 
 struct Caps {
-    uint64_t seg = 0, mat = 0, bkt = 0, part = 0, leaves = 0;
+    uint64_t seg = 0, mat = 0, bkt = 0, part = 0, leaves = 0, fill = 0;
 };
 
 // Device workspace, grown on demand and reused across sorts.
@@ -25,6 +25,7 @@ struct Workspace {
     uint32_t* mat = nullptr;
     uint32_t* part = nullptr;
     BucketRec* bkt = nullptr;
+    FillTask* fill = nullptr;
     LevelCtl* ctl = nullptr;
     SegRec* leaves = nullptr;
     Caps cap;
@@ -38,10 +39,10 @@ struct ProfRec {
 };
 
 // Steps of one partition level, in launch order, and their profiling kinds.
-constexpr int kLevelSteps = 7;
+constexpr int kLevelSteps = 8;
 inline int step_kind(int step) {
-    // splitters, tileplan, count, scan_reduce, scan_apply, plan, scatter
-    static const int kinds[kLevelSteps] = {0, 1, 2, 3, 3, 4, 5};
+    // splitters, tileplan, count, scan_reduce, scan_apply, plan, scatter, fill
+    static const int kinds[kLevelSteps] = {0, 1, 2, 3, 3, 4, 5, 8};
     return kinds[step];
 }
 hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st);

@@ -8,9 +8,9 @@
 namespace gqs {
 
 // Per-bucket result of one partition level: where the bucket starts inside
-// its segment and how many keys it holds.  Bit 31 of `size` marks a bucket
-// whose keys were written straight to their final place in d (an equality
-// bucket, the reference's pivot fill, or a bucket of at most one key).
+// its segment and how many keys it holds.  Bit 31 of `size` marks an
+// equality bucket: its keys all equal one splitter, the scatter does not store
+// them and k_fill writes the value into d after the scatter has finished.
 struct BucketRec {
     uint32_t rel;
     uint32_t size;
@@ -29,6 +29,7 @@ struct LevelArgs {
     uint32_t* mat;           // count matrix (scanned in place)
     uint32_t* part;          // scan chunk partials
     BucketRec* bkt;          // per bucket of this level
+    FillTask* fill;          // equality-bucket fill tasks of this level
     LevelCtl* ctl;           // this level
     LevelCtl* ctl_next;      // next level
     uint32_t mode;           // Mode
@@ -40,6 +41,7 @@ struct LevelArgs {
     uint32_t seg_cap;        // capacity of next_segs
     uint32_t mat_cap;        // capacity of mat
     uint32_t bkt_cap;        // capacity of bkt
+    uint32_t fill_cap;       // capacity of fill
     uint32_t seed;           // sample-position hash seed
 };
 

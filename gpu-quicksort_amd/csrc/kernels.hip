@@ -25,8 +25,11 @@
 //   k_scatter    persistent over tiles: classify again, rank inside the tile
 //                with wave ballots + LDS atomics, stage the tile in LDS grouped
 //                by bucket and store coalesced runs (the scatter pass, :195-203).
-//                Equality buckets and single keys go straight to d in caller
-//                representation (the pivot fill, :214-216, done in parallel).
+//                Keys of equality buckets are not stored at all.
+//   k_fill       equality buckets: the splitter value written over the bucket's
+//                range of d, in caller representation, by many work-groups
+//                after the scatter (the pivot fill, :214-216, which the
+//                reference does serially in the last work-group).
 //   k_leaf       persistent over buckets: local sort of every bucket of at most
 //                leaf_max keys in LDS (lqsort_kernel, :267-421), written to d.
 //
@@ -354,12 +357,14 @@ __global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
     __shared__ uint32_t wsum[kPlanWG / 64];
     const uint32_t nseg = a.ctl->nseg;
     uint32_t ct = 0, cm = 0, cb = 0;
+    uint64_t keys = 0;
     for (uint32_t base = 0; base < nseg; base += kPlanWG) {
         const uint32_t s = base + threadIdx.x;
         uint32_t nt = 0, nb = 0;
         if (s < nseg) {
             nt = a.plans[s].ntiles;
             nb = a.plans[s].nbuckets;
+            keys += a.segs[s].n;
         }
         uint32_t tt, tm, tb;
         const uint32_t et = block_excl_scan<kPlanWG>(nt, wsum, &tt);
@@ -374,7 +379,15 @@ __global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
         cm += tm;
         cb += tb;
     }
+    // per-thread key counts summed through LDS atomics (statistics only)
+    __shared__ unsigned long long skeys;
+    if (threadIdx.x == 0) skeys = 0;
+    __syncthreads();
+    if (keys) atomicAdd(&skeys, (unsigned long long)keys);
+    __syncthreads();
     if (threadIdx.x == 0) {
+        a.ctl->keys_lo = (uint32_t)skeys;
+        a.ctl->keys_hi = (uint32_t)(skeys >> 32);
         if (cm > a.mat_cap || cb > a.bkt_cap) {
             // Capacity is sized from provable bounds; should this ever fire the
             // level is skipped and the host reports GQS_EOVERFLOW.
@@ -499,27 +512,34 @@ __global__ __launch_bounds__(256) void k_scan_apply(LevelArgs a) {
 }
 
 // ------------------------------------------------------------ k_plan
+// One work-group per segment, one thread per bucket (nbuckets <= 256).
 __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
-    __shared__ uint8_t eqb[kSplStride + 1];
+    __shared__ uint16_t eqi[kSplStride + 1];  // splitter index + 1 of an equality bucket, else 0
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t fbase;
     const uint32_t nseg = a.ctl->nseg;
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const SegRec seg = a.segs[s];
         const SegPlan p = a.plans[s];
-        for (uint32_t i = threadIdx.x; i <= (uint32_t)kSplStride; i += 256) eqb[i] = 0;
+        for (uint32_t i = threadIdx.x; i <= (uint32_t)kSplStride; i += 256) eqi[i] = 0;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < (uint32_t)kSplStride; i += 256) {
             const uint16_t m = a.meta[(size_t)s * kSplStride + i];
-            if (m >> 8) eqb[(m & 0xFFu) + 1] = 1;
+            if (m >> 8) eqi[(m & 0xFFu) + 1] = (uint16_t)(i + 1);
         }
         __syncthreads();
         const uint32_t m0 = a.mat[p.mat_base];
-        for (uint32_t b = threadIdx.x; b < p.nbuckets; b += 256) {
-            const uint32_t rel = a.mat[p.mat_base + b * p.ntiles] - m0;
+        const uint32_t b = threadIdx.x;
+        uint32_t rel = 0, size = 0, nfill = 0;
+        bool eq = false;
+        if (b < p.nbuckets) {
+            rel = a.mat[p.mat_base + b * p.ntiles] - m0;
             const uint32_t rel1 = (b + 1 < p.nbuckets) ? a.mat[p.mat_base + (b + 1) * p.ntiles] - m0 : seg.n;
-            const uint32_t size = rel1 - rel;
-            const bool direct = eqb[b] || size <= 1;
-            a.bkt[p.bucket_base + b] = BucketRec{rel, size | (direct ? kBucketDirect : 0u)};
-            if (!direct && size > a.leaf_max && !a.is_last) {
+            size = rel1 - rel;
+            eq = eqi[b] != 0 && size > 0;
+            a.bkt[p.bucket_base + b] = BucketRec{rel, size | (eq ? kBucketDirect : 0u)};
+            if (eq) nfill = (size + kFillChunk - 1) / kFillChunk;
+            if (!eq && size > a.leaf_max && !a.is_last) {
                 const uint32_t slot = atomicAdd(&a.ctl_next->nseg, 1u);
                 if (slot < a.seg_cap) {
                     a.next_segs[slot] = SegRec{seg.start + rel, size, (seg.flags & SEG_SRC_DN) ^ SEG_SRC_DN};
@@ -528,7 +548,38 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
                 }
             }
         }
+        uint32_t ftotal;
+        const uint32_t fex = block_excl_scan<256>(nfill, wsum, &ftotal);
+        if (threadIdx.x == 0 && ftotal) fbase = atomicAdd(&a.ctl->nfill, ftotal);
         __syncthreads();
+        if (nfill) {
+            const uint32_t spl_index = (uint32_t)s * kSplStride + (eqi[b] - 1u);
+            for (uint32_t c = 0; c < nfill; ++c) {
+                const uint32_t slot = fbase + fex + c;
+                const uint32_t off = c * kFillChunk;
+                if (slot < a.fill_cap)
+                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), spl_index};
+                else
+                    atomicOr(&a.ctl->overflow, 4u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ k_fill
+// Equality buckets: write the splitter value, in caller representation, over
+// each task's range of d.  Runs after the level's scatter, so no tile of the
+// level still reads these positions.
+template <int DT>
+__global__ __launch_bounds__(256) void k_fill(LevelArgs a) {
+    using K = typename KeyOf<DT>::K;
+    const uint32_t nt = min(a.ctl->nfill, a.fill_cap);
+    K* d = (K*)a.d;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const FillTask f = a.fill[t];
+        const K v = from_key<DT>(((const K*)a.spl)[f.spl_index]);
+        for (uint32_t i = threadIdx.x; i < f.n; i += 256) d[f.start + i] = v;
     }
 }
 
@@ -566,7 +617,6 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
         }
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         K* dst = (seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn;
-        K* dfinal = (K*)a.d;
         const bool raw = seg.flags & SEG_RAW;
         const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
         const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
@@ -610,10 +660,9 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
             __syncthreads();
             for (uint32_t i = tid; i < cnt; i += kPartWG) {
                 const uint32_t b = sbk[i];
+                if (direct[b]) continue;  // equality bucket: filled by k_fill
                 const uint64_t pos = seg.start + cursor[b] + (i - lstart[b]);
-                const K key = stage[i];
-                if (direct[b]) dfinal[pos] = from_key<DT>(key);
-                else dst[pos] = key;
+                dst[pos] = stage[i];
             }
             __syncthreads();
             if (tid < p.nbuckets) cursor[tid] += hv;
@@ -707,6 +756,7 @@ static hipError_t level_step_t(int step, const LevelArgs& a, const Grid& g, hipS
         case 4: hipLaunchKernelGGL(k_scan_apply, dim3(g.scan), dim3(256), 0, st, a); break;
         case 5: hipLaunchKernelGGL(k_plan, dim3(g.plan), dim3(256), 0, st, a); break;
         case 6: hipLaunchKernelGGL(k_scatter<DT>, dim3(g.part), dim3(kPartWG), 0, st, a); break;
+        case 7: hipLaunchKernelGGL(k_fill<DT>, dim3(g.plan), dim3(256), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

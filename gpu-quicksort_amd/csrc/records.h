@@ -57,7 +57,20 @@ struct LevelCtl {
     uint32_t nleaf_list;     // explicit leaf records (level 0 only)
     uint32_t oversize;       // leaves sorted by the unbounded fallback path
     uint32_t overflow;       // capacity overflow flags (should stay 0)
+    uint32_t nfill;          // equality-bucket fill tasks of this level
+    uint32_t keys_lo;        // keys in this level's segments (low / high word)
+    uint32_t keys_hi;
 };
+
+// One chunk of an equality bucket to fill in d with its key value (the
+// reference's pivot fill, CL12/QuicksortKernels.cl:214-216, split so that
+// many work-groups share a large bucket).
+struct FillTask {
+    uint64_t start;
+    uint32_t n;
+    uint32_t spl_index;  // index of the key value in the splitter table
+};
+constexpr uint32_t kFillChunk = 65536;
 
 constexpr int kMaxSplitters = 255;
 constexpr int kSplStride = 256;  // splitter-table stride per segment

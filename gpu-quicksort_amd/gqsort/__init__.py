@@ -35,7 +35,7 @@ GEN_UNIFORM, GEN_GAUSSIAN, GEN_SORTED, GEN_REVERSE, GEN_EQUAL, GEN_BUCKET, GEN_P
 GENERATORS = {"uniform": 0, "gaussian": 1, "sorted": 2, "reverse": 3, "equal": 4, "bucket": 5,
               "permutation": 6, "normal": 7}
 
-KERNEL_KINDS = ["splitters", "tileplan", "count", "scan", "plan", "scatter", "leaf", "init"]
+KERNEL_KINDS = ["splitters", "tileplan", "count", "scan", "plan", "scatter", "leaf", "init", "fill"]
 
 
 class GQSError(RuntimeError):
@@ -57,7 +57,7 @@ class Params(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("levels_launched", ctypes.c_uint32), ("levels_used", ctypes.c_uint32),
                 ("oversize_leaves", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
-                ("segments", ctypes.c_uint32 * 8)]
+                ("segments", ctypes.c_uint32 * 8), ("level_keys", ctypes.c_uint64 * 8)]
 
 
 # Every symbol include/gqsort.h declares, with its ctypes signature.
@@ -79,6 +79,19 @@ _SIGS = {
     "gqs_device_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_size_t]),
     "gqs_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _P, ctypes.c_size_t]),
+    "gqs_malloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "gqs_free": (ctypes.c_int, [_P, _P]),
+    "gqs_host_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "gqs_host_free": (ctypes.c_int, [_P, _P]),
+    "gqs_memcpy": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P]),
+    "gqs_memcpy_async": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P]),
+    "gqs_stream_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "gqs_stream_destroy": (ctypes.c_int, [_P, _P]),
+    "gqs_stream_sync": (ctypes.c_int, [_P, _P]),
+    "gqs_event_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "gqs_event_destroy": (ctypes.c_int, [_P, _P]),
+    "gqs_event_record": (ctypes.c_int, [_P, _P, _P]),
+    "gqs_event_elapsed": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_float)]),
 }
 OPTIONAL_SIGS = {}
 
@@ -102,9 +115,11 @@ def load(path=LIB_PATH):
 
 
 def _ptr(x):
-    """Device pointer of a torch tensor, or an int address."""
+    """Device address of a DeviceBuffer, a torch tensor, or an int address."""
     if x is None:
         return None
+    if hasattr(x, "ptr"):
+        return x.ptr
     if hasattr(x, "data_ptr"):
         return x.data_ptr()
     return int(x)
@@ -113,6 +128,8 @@ def _ptr(x):
 def _stream(s):
     if s is None:
         return None
+    if hasattr(s, "handle") and isinstance(s, Stream):
+        return s.handle
     if hasattr(s, "cuda_stream"):
         return s.cuda_stream
     return int(s)
@@ -232,3 +249,97 @@ def generate(dtype, gen, seed, n):
     a = np.empty(n, dtype=NP_DTYPES[dtype])
     _check(load().gqs_generate(dtype, gen, seed, a.ctypes.data if n else None, n))
     return a
+
+
+class DeviceBuffer:
+    """Device memory from the library's own HIP runtime."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = _P()
+        _check(ctx._lib.gqs_malloc(ctx.handle, nbytes, ctypes.byref(p)), ctx)
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx, a, stream=None):
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(self.ctx._lib.gqs_memcpy(self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes, _stream(stream)),
+               self.ctx)
+
+    def download(self, dtype, count, stream=None, offset_bytes=0):
+        out = np.empty(count, dtype=dtype)
+        _check(self.ctx._lib.gqs_memcpy(self.ctx.handle, out.ctypes.data, self.ptr + offset_bytes, out.nbytes,
+                                        _stream(stream)), self.ctx)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._lib.gqs_free(self.ctx.handle, self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            if self.ctx._h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream of the library's runtime."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = _P()
+        _check(ctx._lib.gqs_stream_create(ctx.handle, ctypes.byref(h)), ctx)
+        self.handle = h.value
+
+    def sync(self):
+        _check(self.ctx._lib.gqs_stream_sync(self.ctx.handle, self.handle), self.ctx)
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.gqs_stream_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = _P()
+        _check(ctx._lib.gqs_event_create(ctx.handle, ctypes.byref(h)), ctx)
+        self.handle = h.value
+
+    def record(self, stream=None):
+        _check(self.ctx._lib.gqs_event_record(self.ctx.handle, self.handle, _stream(stream)), self.ctx)
+
+    def elapsed_ms(self, stop):
+        ms = ctypes.c_float()
+        _check(self.ctx._lib.gqs_event_elapsed(self.ctx.handle, self.handle, stop.handle, ctypes.byref(ms)),
+               self.ctx)
+        return float(ms.value)
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.gqs_event_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+def device_sort(ctx, a, stream=None):
+    """Upload host array a, sort it on the device, download the result."""
+    dt = DTYPE_CODES[a.dtype]
+    d = DeviceBuffer.from_array(ctx, a, stream)
+    dn = DeviceBuffer(ctx, a.nbytes)
+    ctx.sort_device(dt, d, dn, a.size, stream)
+    out = d.download(a.dtype, a.size, stream)
+    d.free()
+    dn.free()
+    return out

@@ -70,10 +70,11 @@ typedef struct gqs_params {
 /* Counters of the last sort on a context. */
 typedef struct gqs_stats {
     uint32_t levels_launched;   /* partition levels enqueued */
-    uint32_t levels_used;       /* levels that had segments (host_driven or after gqs_sync_stats) */
+    uint32_t levels_used;       /* levels that had segments */
     uint32_t oversize_leaves;   /* leaves sorted by the unbounded fallback */
     uint32_t overflow;          /* non-zero if a queue bound was hit */
     uint32_t segments[8];       /* segments per level (first 8 levels) */
+    uint64_t level_keys[8];     /* keys partitioned per level (first 8 levels) */
 } gqs_stats;
 
 /* Context: owns the device, its workspace (grown on demand) and parameters. */
@@ -104,7 +105,7 @@ gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
 /* Per-kernel timing with HIP events on the sort's own stream.  While enabled,
  * every launch is bracketed by events; gqs_kernel_times() synchronises and
  * returns, per kernel kind, the number of launches and total milliseconds.
- * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init. */
+ * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init, 8 fill. */
 gqs_status gqs_set_profiling(gqs_ctx* ctx, int on);
 gqs_status gqs_kernel_times(gqs_ctx* ctx, uint32_t* launches, double* ms, int nkinds);
 
@@ -118,6 +119,23 @@ gqs_status gqs_device_info(gqs_ctx* ctx, int* cus, int* lds_per_wg, int* wave_si
  * 1..n (the reference's own data, CL12/QuicksortMain.cpp:415-416),
  * 7 normal floats with 1% wide outliers and 0.1% duplicates. */
 gqs_status gqs_generate(int dtype, int gen, uint64_t seed, void* out, size_t n);
+
+/* ---- Device plumbing from the library's own HIP runtime -----------------
+ * For callers behind a foreign-function layer.  Streams and events are
+ * hipStream_t / hipEvent_t passed as void*. */
+gqs_status gqs_malloc(gqs_ctx* ctx, size_t bytes, void** out);
+gqs_status gqs_free(gqs_ctx* ctx, void* p);
+gqs_status gqs_host_alloc(gqs_ctx* ctx, size_t bytes, void** out); /* pinned host memory */
+gqs_status gqs_host_free(gqs_ctx* ctx, void* p);
+gqs_status gqs_memcpy(gqs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);       /* synchronous */
+gqs_status gqs_memcpy_async(gqs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream); /* stream-ordered */
+gqs_status gqs_stream_create(gqs_ctx* ctx, void** out);
+gqs_status gqs_stream_destroy(gqs_ctx* ctx, void* stream);
+gqs_status gqs_stream_sync(gqs_ctx* ctx, void* stream);
+gqs_status gqs_event_create(gqs_ctx* ctx, void** out);
+gqs_status gqs_event_destroy(gqs_ctx* ctx, void* ev);
+gqs_status gqs_event_record(gqs_ctx* ctx, void* ev, void* stream);
+gqs_status gqs_event_elapsed(gqs_ctx* ctx, void* start, void* stop, float* ms); /* waits for stop */
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,121 @@
+"""C ABI and host-side logic of libgqsort.so (CPU only: no kernel launches)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gqsort
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for name in os.listdir(os.path.join(ROOT, "include")):
+        if name.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", name)).read()
+            syms |= set(re.findall(r"^\s*(?:gqs_status|const char\*|void)\s+(gqs_\w+)\s*\(", text, re.M))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(gqsort.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert declared_symbols() <= set(gqsort._SIGS) | set(gqsort.OPTIONAL_SIGS)
+
+
+def test_default_params():
+    p = gqsort.default_params(gqsort.U32)
+    assert p.mode == gqsort.MODE_MULTIPIVOT and p.max_fanout == 256
+    assert 16 <= p.leaf_target <= p.leaf_max <= 16384
+    q = gqsort.default_params(gqsort.F64)
+    assert q.leaf_max * 8 <= 65536
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-GPU error path")
+def test_create_without_gpu_reports_error():
+    h = ctypes.c_void_p()
+    st = gqsort.load().gqs_create(ctypes.byref(h), 0)
+    assert st == 3 and not h.value  # GQS_EDEVICE, no context
+
+
+def test_null_arguments_are_rejected():
+    lib = gqsort.load()
+    assert lib.gqs_sort(None, 0, None, None, 10) == 1
+    assert lib.gqs_sort_device(None, 0, None, None, 10, None) == 1
+    assert lib.gqs_default_params(0, None) == 1
+    assert lib.gqs_generate(0, 0, 1, None, 10) == 1
+    assert lib.gqs_generate(0, 99, 1, None, 0) == 0
+
+
+# ------------------------------------------------------------- generators
+N = 1 << 16
+
+
+def test_generators_deterministic_and_seeded():
+    for gen in range(7):
+        a = gqsort.generate(gqsort.U32, gen, 1, N)
+        assert np.array_equal(a, gqsort.generate(gqsort.U32, gen, 1, N))
+        if gen != 4:
+            assert not np.array_equal(a, gqsort.generate(gqsort.U32, gen, 2, N))
+
+
+def test_uniform_and_gaussian_moments():
+    u = gqsort.generate(gqsort.U32, "uniform", 1, 1 << 20).astype(np.float64)
+    g = gqsort.generate(gqsort.U32, "gaussian", 1, 1 << 20).astype(np.float64)
+    m = 2.0**31
+    assert abs(u.mean() - m) / m < 0.01 and abs(g.mean() - m) / m < 0.01
+    # mean of 4 uniforms: standard deviation halves
+    assert abs(g.std() / u.std() - 0.5) < 0.02
+
+
+def test_sorted_reverse_equal():
+    s = gqsort.generate(gqsort.U32, "sorted", 3, N)
+    r = gqsort.generate(gqsort.U32, "reverse", 3, N)
+    assert np.all(np.diff(s.astype(np.int64)) > 0)
+    assert np.array_equal(r, s[::-1])
+    assert np.all(gqsort.generate(gqsort.U32, "equal", 3, N) == 0x9E3779B9)
+
+
+def test_bucket_runs():
+    n = 256 * 1000
+    a = gqsort.generate(gqsort.U32, "bucket", 4, n)
+    runs = a.reshape(256, -1) >> 24
+    assert np.all(runs == runs[:, :1])  # each run inside one 2^24-wide range
+    assert sorted(runs[:, 0].tolist()) == list(range(256))  # every range once
+    assert not np.array_equal(runs[:, 0], np.arange(256))  # shuffled
+
+
+def test_permutation_is_reference_data():
+    a = gqsort.generate(gqsort.U32, "permutation", 1, N)
+    assert np.array_equal(np.sort(a), np.arange(1, N + 1, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("dt,npdt", [(gqsort.F32, np.float32), (gqsort.F64, np.float64)])
+def test_float_generator(dt, npdt):
+    n = 1 << 20
+    a = gqsort.generate(dt, "normal", 5, n)
+    assert a.dtype == npdt
+    assert np.all(np.isfinite(a))
+    assert not np.any((a == 0) & np.signbit(a))  # no -0
+    wide = np.abs(a) > 1e6
+    assert 0.008 < wide.mean() < 0.012  # ~1% outliers in [-1e30, 1e30]
+    body = a[~wide].astype(np.float64)
+    assert abs(body.mean()) < 0.01 and abs(body.std() - 1) < 0.01
+    _, counts = np.unique(a, return_counts=True)
+    assert (counts > 1).sum() > n * 0.0005  # ~0.1% duplicated values
+
+
+def test_generator_rejects_bad_combinations():
+    lib = gqsort.load()
+    buf = np.empty(16, np.uint32)
+    assert lib.gqs_generate(gqsort.U32, 7, 1, buf.ctypes.data, 16) == 1
+    assert lib.gqs_generate(gqsort.F32, 0, 1, buf.ctypes.data, 16) == 1

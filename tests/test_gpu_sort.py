@@ -88,7 +88,6 @@ def test_host_pointer_entry(gpu_ctx):
 
 @pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
 def test_large_2p24_properties(gpu_ctx, gen):
-    import torch
 
     n = 1 << 24
     a = gqsort.generate(gqsort.U32, gen, 21, n)
