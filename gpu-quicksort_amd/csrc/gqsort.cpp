@@ -167,7 +167,8 @@ uint32_t levels_needed(uint64_t n, const gqs_params& p) {
 gqs_status Ctx::run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets, uint32_t nseg_in,
                     hipStream_t st) {
     gqs_params p = params;
-    const uint32_t cap = (dtype == DT_F64) ? 8192u : 16384u;
+    // a work-group leaf holds 32 (u32/f32) or 16 (f64) keys per thread
+    const uint32_t cap = (dtype == DT_F64) ? 4096u : 8192u;
     p.leaf_max = std::min(p.leaf_max, cap);
     if (gqs_status s = ensure_workspace(n, nseg_in, p, dtype)) return s;
     last_stream = st;
@@ -369,7 +370,7 @@ gqs_status gqs_set_params(gqs_ctx* c, const gqs_params* p) {
     std::lock_guard<std::mutex> lk(c->mu);
     const bool pow2 = p->max_fanout >= 2 && p->max_fanout <= 256 && (p->max_fanout & (p->max_fanout - 1)) == 0;
     if ((p->mode != GQS_MODE_MULTIPIVOT && p->mode != GQS_MODE_CLASSIC) || p->leaf_max < 16 ||
-        p->leaf_max > 16384 || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
+        p->leaf_max > 8192 || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
         p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32)
         return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");
     c->params = *p;
@@ -452,17 +453,32 @@ gqs_status gqs_kernel_times(gqs_ctx* c, uint32_t* launches, double* ms, int nkin
         ms[k] = 0.0;
     }
     if (c->fail(hipStreamSynchronize(c->last_stream), "sync")) return GQS_EDEVICE;
+    c->trace.clear();
     for (auto& r : c->prof) {
         float t = 0.f;
-        if (r.kind < nkinds && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
-            launches[r.kind]++;
-            ms[r.kind] += t;
+        if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            c->trace.emplace_back(r.kind, t);
+            if (r.kind < nkinds) {
+                launches[r.kind]++;
+                ms[r.kind] += t;
+            }
         }
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
     }
     c->prof.clear();
     return GQS_OK;
+}
+
+int gqs_kernel_trace(gqs_ctx* c, int* kinds, float* ms, int cap) {
+    if (!c || cap < 0 || (cap > 0 && (!kinds || !ms))) return -1;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const int n = (int)c->trace.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        kinds[i] = c->trace[i].first;
+        ms[i] = c->trace[i].second;
+    }
+    return n;
 }
 
 gqs_status gqs_device_info(gqs_ctx* c, int* cus, int* lds_per_wg, int* wave_size, char* name, size_t name_len) {

@@ -64,6 +64,7 @@ struct Ctx {
     gqs_stats stats{};
     bool profiling = false;
     std::vector<ProfRec> prof;
+    std::vector<std::pair<int, float>> trace;  // per launch of the last profiled sort
     // device copies for the host-pointer entry point
     void* host_d = nullptr;
     void* host_dn = nullptr;

@@ -12,8 +12,9 @@ namespace gqs {
 // equality bucket: its keys all equal one splitter, the scatter does not store
 // them and k_fill writes the value into d after the scatter has finished.
 struct BucketRec {
-    uint32_t rel;
-    uint32_t size;
+    uint64_t start;   // absolute index of the bucket's first key
+    uint32_t size;    // keys (bit 31: equality bucket)
+    uint32_t in_dn;   // 1 when the bucket's keys were written to dn
 };
 constexpr uint32_t kBucketDirect = 0x80000000u;
 

@@ -31,7 +31,9 @@
 //                after the scatter (the pivot fill, :214-216, which the
 //                reference does serially in the last work-group).
 //   k_leaf       persistent over buckets: local sort of every bucket of at most
-//                leaf_max keys in LDS (lqsort_kernel, :267-421), written to d.
+//                leaf_max keys (lqsort_kernel, :267-421): a local k-way
+//                partition into LDS, then wave-register bitonic tails, written
+//                to d in caller representation.
 //
 // Keys are sorted as unsigned integers: float/double bits go through an
 // order-preserving bijection on the first load and back on the final store.
@@ -147,6 +149,42 @@ __device__ __forceinline__ uint32_t classify(K key, const K* sK, const uint16_t*
     for (uint32_t step = (1u << depth) >> 1; step > 0; step >>= 1) i += (sK[i + step - 1] < key) ? step : 0u;
     const uint32_t meta = sMeta[i];
     return (meta & 0xFFu) + (((meta >> 8) & 1u) & (uint32_t)(key == sK[i]));
+}
+
+// classify() for N keys at once with a compile-time search depth: the N
+// searches advance step by step together, so their LDS reads overlap.
+template <class K, int D, int N>
+__device__ __forceinline__ void classify_n_d(const K (&key)[N], uint32_t (&b)[N], const K* sK,
+                                             const uint16_t* sMeta) {
+    uint32_t i[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) i[j] = 0;
+#pragma unroll
+    for (int st = D - 1; st >= 0; --st) {
+        const uint32_t step = 1u << st;
+#pragma unroll
+        for (int j = 0; j < N; ++j) i[j] += (sK[i[j] + step - 1] < key[j]) ? step : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint32_t meta = sMeta[i[j]];
+        b[j] = (meta & 0xFFu) + (((meta >> 8) & 1u) & (uint32_t)(key[j] == sK[i[j]]));
+    }
+}
+template <class K, int N>
+__device__ __forceinline__ void classify_n(const K (&key)[N], uint32_t (&b)[N], const K* sK, const uint16_t* sMeta,
+                                           uint32_t depth) {
+    switch (depth) {
+        case 0: classify_n_d<K, 0, N>(key, b, sK, sMeta); break;
+        case 1: classify_n_d<K, 1, N>(key, b, sK, sMeta); break;
+        case 2: classify_n_d<K, 2, N>(key, b, sK, sMeta); break;
+        case 3: classify_n_d<K, 3, N>(key, b, sK, sMeta); break;
+        case 4: classify_n_d<K, 4, N>(key, b, sK, sMeta); break;
+        case 5: classify_n_d<K, 5, N>(key, b, sK, sMeta); break;
+        case 6: classify_n_d<K, 6, N>(key, b, sK, sMeta); break;
+        case 7: classify_n_d<K, 7, N>(key, b, sK, sMeta); break;
+        default: classify_n_d<K, 8, N>(key, b, sK, sMeta); break;
+    }
 }
 
 // Largest s < count with field(s) <= t, for strictly increasing fields.
@@ -432,8 +470,24 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = a.ctl->nseg;
     uint32_t cur = 0xFFFFFFFFu;
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        if (tid == 0) L.seg = upper_find(nseg, t, [&](uint32_t i) { return a.plans[i].tile_begin; });
+    // Each work-group takes a contiguous run of tiles, so consecutive tiles
+    // mostly share a segment (splitters stay in LDS) and thread 0 advances the
+    // segment index instead of searching for it.
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
+    uint32_t seg_i = 0, seg_next = 0;
+    if (tid == 0 && t0 < t1) {
+        seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
+        seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+    }
+    for (uint32_t t = t0; t < t1; ++t) {
+        if (tid == 0) {
+            while (t >= seg_next) {
+                ++seg_i;
+                seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+            }
+            L.seg = seg_i;
+        }
         __syncthreads();
         const uint32_t s = L.seg;
         if (s != cur) {
@@ -457,10 +511,12 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
                 const uint32_t idx = sub + j * kPartWG + tid;
                 keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
             }
+            uint32_t bk[TS::kKeysPerThread];
+            classify_n<K, TS::kKeysPerThread>(keys, bk, L.sK, L.sMeta, p.depth);
 #pragma unroll
             for (int j = 0; j < TS::kKeysPerThread; ++j) {
                 const uint32_t idx = sub + j * kPartWG + tid;
-                if (idx < hi) hist_add(L.hist, classify<K>(keys[j], L.sK, L.sMeta, p.depth));
+                if (idx < hi) hist_add(L.hist, bk[j]);
             }
         }
         __syncthreads();
@@ -537,7 +593,8 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
             const uint32_t rel1 = (b + 1 < p.nbuckets) ? a.mat[p.mat_base + (b + 1) * p.ntiles] - m0 : seg.n;
             size = rel1 - rel;
             eq = eqi[b] != 0 && size > 0;
-            a.bkt[p.bucket_base + b] = BucketRec{rel, size | (eq ? kBucketDirect : 0u)};
+            a.bkt[p.bucket_base + b] =
+                BucketRec{seg.start + rel, size | (eq ? kBucketDirect : 0u), (seg.flags & SEG_SRC_DN) ? 0u : 1u};
             if (eq) nfill = (size + kFillChunk - 1) / kFillChunk;
             if (!eq && size > a.leaf_max && !a.is_last) {
                 const uint32_t slot = atomicAdd(&a.ctl_next->nseg, 1u);
@@ -599,8 +656,24 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = a.ctl->nseg;
     uint32_t cur = 0xFFFFFFFFu;
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        if (tid == 0) L.seg = upper_find(nseg, t, [&](uint32_t i) { return a.plans[i].tile_begin; });
+    // Each work-group takes a contiguous run of tiles, so consecutive tiles
+    // mostly share a segment (splitters stay in LDS) and thread 0 advances the
+    // segment index instead of searching for it.
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
+    uint32_t seg_i = 0, seg_next = 0;
+    if (tid == 0 && t0 < t1) {
+        seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
+        seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+    }
+    for (uint32_t t = t0; t < t1; ++t) {
+        if (tid == 0) {
+            while (t >= seg_next) {
+                ++seg_i;
+                seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+            }
+            L.seg = seg_i;
+        }
         __syncthreads();
         const uint32_t s = L.seg;
         if (s != cur) {
@@ -632,15 +705,12 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
                 const uint32_t idx = sub + j * kPartWG + tid;
                 keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
             }
+            classify_n<K, TS::kKeysPerThread>(keys, bk, L.sK, L.sMeta, p.depth);
 #pragma unroll
             for (int j = 0; j < TS::kKeysPerThread; ++j) {
                 const uint32_t idx = sub + j * kPartWG + tid;
-                bk[j] = 0;
                 rk[j] = 0;
-                if (idx < hi) {
-                    bk[j] = classify<K>(keys[j], L.sK, L.sMeta, p.depth);
-                    rk[j] = hist_rank(L.hist, bk[j]);
-                }
+                if (idx < hi) rk[j] = hist_rank(L.hist, bk[j]);
             }
             __syncthreads();
             uint32_t cnt;
@@ -672,62 +742,366 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
 }
 
 // ------------------------------------------------------------ k_leaf
-// One work-group per leaf; LDS holds the leaf (dynamic size leaf_max keys).
-template <int DT>
-__global__ __launch_bounds__(kLeafWG) void k_leaf(LeafArgs a) {
+// Local sort of one leaf (the reference's lqsort_kernel,
+// CL12/QuicksortKernels.cl:267-421, with sort_threshold/bitonic_sort as its
+// tail, :40-97), reshaped for wave64:
+//   * leaves of at most kWaveSortMax keys: one wave sorts the leaf in its
+//     registers (lane-major bitonic network, cross-lane steps by shuffles);
+//     the four waves of a work-group take four such leaves at once;
+//   * larger leaves: one k-way partition of the whole leaf by the work-group
+//     (sampled splitters, keys held in registers, ranks from wave-aggregated
+//     LDS atomics, keys placed by bucket in LDS), then every sub-bucket is a
+//     bitonic tail for one wave; equality sub-buckets are filled with their
+//     value; a sub-bucket still above kWaveSortMax (rare) is sorted in LDS by
+//     the whole work-group.
+constexpr int kWaveSortMax = 512;
+constexpr int kLeafSplit = 64;  // largest local fan-out
+constexpr int kLeafBigWG = 512;
+
+template <class K>
+__device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
+template <class K>
+__device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
+
+// Value of `x` held by lane (lane ^ lm).  The masks the bitonic network needs
+// map to DPP row/quad permutes where one exists (quad_perm for 1, 2, 3;
+// row_half_mirror for 7; row_ror:8 for 8; row_mirror for 15; two row_ror
+// for 4), to ds_swizzle within 32 lanes for 16 and 31, and to ds_bpermute
+// for 32 and 63.  After unrolling, lm is a constant and the switch folds.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t xlane32(uint32_t x, uint32_t lm) {
+    switch (lm) {
+        case 1: return dpp32<0xB1>(x);   // quad_perm [1,0,3,2]
+        case 2: return dpp32<0x4E>(x);   // quad_perm [2,3,0,1]
+        case 3: return dpp32<0x1B>(x);   // quad_perm [3,2,1,0]
+        case 4: {                        // row_ror:12 gives lane+4, row_ror:4 gives lane-4
+            const uint32_t up = dpp32<0x12C>(x), dn = dpp32<0x124>(x);
+            return (lane_id() & 4u) ? dn : up;
+        }
+        case 7: return dpp32<0x141>(x);  // row_half_mirror
+        case 8: return dpp32<0x128>(x);  // row_ror:8
+        case 15: return dpp32<0x140>(x); // row_mirror
+        case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);  // xor 16 within 32
+        case 31: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x7C1F);  // xor 31 within 32
+        default: return __shfl_xor(x, lm, 64);
+    }
+}
+template <class K>
+__device__ __forceinline__ K shfl_xor_k(K v, uint32_t mask) {
+    if constexpr (sizeof(K) == 8) {
+        const uint32_t lo = xlane32((uint32_t)v, mask);
+        const uint32_t hi = xlane32((uint32_t)(v >> 32), mask);
+        return ((K)hi << 32) | lo;
+    } else {
+        return xlane32(v, mask);
+    }
+}
+
+template <int E> struct Log2 { static constexpr int v = (E <= 1) ? 0 : 1 + Log2<E / 2>::v; };
+template <> struct Log2<1> { static constexpr int v = 0; };
+
+// Ascending bitonic sort of 64*E keys held lane-major (element i = lane*E + r
+// lives in v[r] of lane i/E).  All comparators ascending (mirror first step).
+template <class K, int E>
+__device__ __forceinline__ void wave_bitonic(K (&v)[E]) {
+    constexpr int LE = Log2<E>::v;
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int m = 1; m <= 6 + LE; ++m) {
+        const int k = 1 << m;
+        if (k <= E) {
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int p = r ^ (k - 1);
+                if (p > r) {
+                    const K lo = kmin(v[r], v[p]), hi = kmax(v[r], v[p]);
+                    v[r] = lo;
+                    v[p] = hi;
+                }
+            }
+        } else {
+            const uint32_t lm = (uint32_t)(k - 1) >> LE;
+            const bool upper = (lane & ((uint32_t)(k >> 1) >> LE)) != 0;
+            K t[E];
+#pragma unroll
+            for (int r = 0; r < E; ++r) t[r] = shfl_xor_k(v[E - 1 - r], lm);
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = upper ? kmax(v[r], t[r]) : kmin(v[r], t[r]);
+        }
+#pragma unroll
+        for (int j = k >> 2; j >= 1; j >>= 1) {
+            if (j < E) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if (!(r & j)) {
+                        const K lo = kmin(v[r], v[r + j]), hi = kmax(v[r], v[r + j]);
+                        v[r] = lo;
+                        v[r + j] = hi;
+                    }
+                }
+            } else {
+                const uint32_t lm = (uint32_t)j >> LE;
+                const bool upper = (lane & lm) != 0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const K t = shfl_xor_k(v[r], lm);
+                    v[r] = upper ? kmax(v[r], t) : kmin(v[r], t);
+                }
+            }
+        }
+    }
+}
+
+// One wave sorts n <= 64*E keys read through `ld(i)` and stores key i with `st(i, key)`.
+template <class K, int E, class LD, class ST>
+__device__ __forceinline__ void wave_sort_n(uint32_t n, LD ld, ST st) {
+    const uint32_t lane = lane_id();
+    K v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const uint32_t i = lane * E + r;
+        v[r] = (i < n) ? ld(i) : key_max<K>();
+    }
+    wave_bitonic<K, E>(v);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const uint32_t i = lane * E + r;
+        if (i < n) st(i, v[r]);
+    }
+}
+
+template <class K, class LD, class ST>
+__device__ __forceinline__ void wave_sort_any(uint32_t n, LD ld, ST st) {
+    if (n <= 64) wave_sort_n<K, 1>(n, ld, st);
+    else if (n <= 128) wave_sort_n<K, 2>(n, ld, st);
+    else if (n <= 256) wave_sort_n<K, 4>(n, ld, st);
+    else wave_sort_n<K, 8>(n, ld, st);
+}
+
+struct LeafDesc {
+    uint64_t start;
+    uint32_t n;
+    uint32_t kind;  // 0 none, 1 wave, 2 work-group, 3 unbounded
+    uint32_t from_dn;
+    uint32_t raw;
+};
+
+// BIG = false: leaves of at most kWaveSortMax keys, one per wave, four per
+// work-group of 256 threads.  BIG = true: larger leaves, one per work-group
+// of kLeafBigWG threads.  Both scan the same leaf space and skip the other's.
+template <int DT, bool BIG>
+__global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_leaf(LeafArgs a) {
     using K = typename KeyOf<DT>::K;
+    constexpr int NT = BIG ? kLeafBigWG : kLeafWG;
+    constexpr int NW = NT / 64;
+    constexpr int KPT = (sizeof(K) == 4) ? 8192 / NT : 4096 / NT;  // keys per thread of a work-group leaf
     extern __shared__ unsigned char leaf_smem[];
-    K* buf = reinterpret_cast<K*>(leaf_smem);
-    __shared__ uint32_t s_seg;
-    const uint32_t tid = threadIdx.x;
+    K* A = reinterpret_cast<K*>(leaf_smem);
+    __shared__ LeafDesc desc[4];
+    __shared__ K sK[kLeafSplit];
+    __shared__ uint16_t sMeta[kLeafSplit];
+    __shared__ uint8_t beq[kLeafSplit + 1];  // splitter index + 1 of an equality sub-bucket
+    __shared__ uint32_t hist[kLeafSplit + 1];
+    __shared__ uint32_t bst[kLeafSplit + 1];
+    __shared__ uint32_t s_nb, s_depth, s_nbig;
+    __shared__ uint8_t bigs[kLeafSplit];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
-    const uint32_t nseg = list_mode ? 0u : a.lv.ctl->nseg;
-    for (uint32_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-        uint64_t start;
-        uint32_t n;
-        bool from_dn, raw;
-        if (list_mode) {
-            const SegRec r = a.list[idx];
-            start = r.start;
-            n = r.n;
-            from_dn = r.flags & SEG_SRC_DN;
-            raw = r.flags & SEG_RAW;
-        } else {
-            if (tid == 0) s_seg = upper_find(nseg, idx, [&](uint32_t i) { return a.lv.plans[i].bucket_base; });
-            __syncthreads();
-            const uint32_t s = s_seg;
-            __syncthreads();
-            const BucketRec br = a.lv.bkt[idx];
-            if (br.size & kBucketDirect) continue;
-            n = br.size;
-            if (n > a.leaf_max && !a.is_last) continue;  // partitioned again at the next level
-            const SegRec seg = a.lv.segs[s];
-            start = seg.start + br.rel;
-            from_dn = !(seg.flags & SEG_SRC_DN);
-            raw = false;
+    K* dout = (K*)a.d;
+    constexpr uint32_t G = BIG ? 1u : 4u;  // leaves looked at per iteration
+    for (uint32_t g = blockIdx.x; g * G < count; g += gridDim.x) {
+        if (tid < G) {
+            LeafDesc ds{0, 0, 0, 0, 0};
+            const uint32_t idx = g * G + tid;
+            if (idx < count) {
+                if (list_mode) {
+                    const SegRec r = a.list[idx];
+                    ds.start = r.start;
+                    ds.n = r.n;
+                    ds.from_dn = (r.flags & SEG_SRC_DN) ? 1u : 0u;
+                    ds.raw = (r.flags & SEG_RAW) ? 1u : 0u;
+                } else {
+                    const BucketRec br = a.lv.bkt[idx];
+                    if (!(br.size & kBucketDirect) && (br.size <= a.leaf_max || a.is_last)) {
+                        ds.start = br.start;
+                        ds.n = br.size;
+                        ds.from_dn = br.in_dn;
+                        ds.raw = 0;
+                    }
+                }
+                if (ds.n >= 1) ds.kind = (ds.n <= (uint32_t)kWaveSortMax) ? 1u : (ds.n <= a.leaf_max ? 2u : 3u);
+                if ((ds.kind == 1) == BIG) ds.kind = 0;  // the other kernel's leaf
+            }
+            desc[tid] = ds;
         }
-        if (n == 0) continue;
-        const K* src = from_dn ? (const K*)a.dn : (const K*)a.d;
-        K* dout = (K*)a.d;
-        if (n <= a.leaf_max) {
-            for (uint32_t i = tid; i < n; i += kLeafWG) buf[i] = load_key<DT>(src, start + i, raw);
-            __syncthreads();
-            lds_bitonic<K>(buf, n, tid, kLeafWG);
-            for (uint32_t i = tid; i < n; i += kLeafWG) dout[start + i] = from_key<DT>(buf[i]);
-            __syncthreads();
-        } else {
-            // Unbounded fallback: sort in place in the source buffer.
-            K* g = (K*)src + start;
-            if (raw)
-                for (uint32_t i = tid; i < n; i += kLeafWG) g[i] = to_key<DT>(g[i]);
-            __threadfence_block();
-            __syncthreads();
-            global_bitonic<K>(g, n, tid, kLeafWG);
-            for (uint32_t i = tid; i < n; i += kLeafWG) dout[start + i] = from_key<DT>(g[i]);
-            if (tid == 0) atomicAdd(&a.lv.ctl->oversize, 1u);
-            __syncthreads();
+        __syncthreads();
+        // Small leaves: one per wave, entirely in registers.
+        if constexpr (!BIG) {
+            const LeafDesc ds = desc[wave];
+            if (ds.kind == 1) {
+                const K* src = (ds.from_dn ? (const K*)a.dn : (const K*)a.d) + ds.start;
+                K* out = dout + ds.start;
+                const bool raw = ds.raw;
+                wave_sort_any<K>(
+                    ds.n, [&](uint32_t i) { return load_key<DT>(src, i, raw); },
+                    [&](uint32_t i, K v) { out[i] = from_key<DT>(v); });
+            }
         }
+        // Large leaves: the whole work-group.
+        if constexpr (BIG) {
+            const LeafDesc ds = desc[0];
+            if (ds.kind < 2) {
+                __syncthreads();  // desc[] is rewritten by the next iteration
+                continue;
+            }
+            const K* src = (ds.from_dn ? (const K*)a.dn : (const K*)a.d) + ds.start;
+            K* out = dout + ds.start;
+            const bool raw = ds.raw;
+            const uint32_t n = ds.n;
+            if (ds.kind == 3) {
+                // Unbounded fallback: bitonic network in place in the source buffer.
+                K* gsrc = (K*)src;
+                if (raw)
+                    for (uint32_t i = tid; i < n; i += NT) gsrc[i] = to_key<DT>(gsrc[i]);
+                __threadfence_block();
+                __syncthreads();
+                global_bitonic<K>(gsrc, n, tid, NT);
+                for (uint32_t i = tid; i < n; i += NT) out[i] = from_key<DT>(gsrc[i]);
+                if (tid == 0) atomicAdd(&a.lv.ctl->oversize, 1u);
+                __syncthreads();
+                continue;
+            }
+            // 1. the leaf's keys into registers (loads in flight during step 2)
+            K kv[KPT];
+            uint32_t kbr[KPT];  // bucket | rank << 8
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t i = j * NT + tid;
+                kbr[j] = 0;
+                kv[j] = key_max<K>();
+                if (i < n) kv[j] = load_key<DT>(src, i, raw);
+            }
+            // 2. splitters from a stratified sample, sorted by wave 0
+            uint32_t k = 2;
+            while (k < kLeafSplit && k * 256 < n) k <<= 1;
+            const uint32_t S = min((uint32_t)kWaveSortMax, 16u * k);
+            if (tid <= (uint32_t)kLeafSplit) hist[tid] = 0;
+            if (tid < S) {
+                const uint64_t h = hash32(0x1eafu, (uint32_t)ds.start, tid) % n;
+                A[tid] = load_key<DT>(src, ((uint64_t)tid * n + h) / S, raw);
+            }
+            __syncthreads();
+            if (wave == 0) {
+                K v[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const uint32_t j = lane * 8 + r;
+                    v[r] = (j < S) ? A[j] : key_max<K>();
+                }
+                wave_bitonic<K, 8>(v);
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int r = 0; r < 8; ++r) A[lane * 8 + r] = v[r];
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // candidate i (i < k-1) in lane i; unique splitters with equality flags
+                const uint32_t nc = k - 1;
+                K c = key_max<K>();
+                if (lane < nc) c = A[((lane + 1) * S) / k];
+                const K cprev = __shfl_up(c, 1, 64);
+                const K cnext = __shfl_down(c, 1, 64);
+                const bool first = lane < nc && (lane == 0 || cprev != c);
+                const bool eq = first && lane + 1 < nc && cnext == c;
+                const uint64_t fm = __ballot(first), em = __ballot(eq);
+                const uint32_t m = (uint32_t)__popcll(fm), neq = (uint32_t)__popcll(em);
+                const uint32_t uidx = mbcnt(fm), ebefore = mbcnt(em);
+                beq[lane] = 0;
+                if (lane == 0) beq[kLeafSplit] = 0;
+                __builtin_amdgcn_wave_barrier();
+                if (first) {
+                    sK[uidx] = c;
+                    sMeta[uidx] = (uint16_t)((uidx + ebefore) | ((eq ? 1u : 0u) << 8));
+                    if (eq) beq[uidx + ebefore + 1] = (uint8_t)(uidx + 1);
+                }
+                const uint32_t nb = m + 1 + neq;
+                if (lane >= m) {
+                    sK[lane] = key_max<K>();
+                    sMeta[lane] = (uint16_t)((lane == m) ? (m + neq) : (nb - 1));
+                }
+                if (lane == 0) {
+                    uint32_t depth = 0;
+                    while (((1u << depth) - 1u) < m) ++depth;
+                    s_nb = nb;
+                    s_depth = depth;
+                    s_nbig = 0;
+                }
+            }
+            __syncthreads();
+            const uint32_t nb = s_nb, depth = s_depth;
+            // classify and rank
+            {
+                uint32_t bj[KPT];
+                classify_n<K, KPT>(kv, bj, sK, sMeta, depth);
+#pragma unroll
+                for (int j = 0; j < KPT; ++j) {
+                    const uint32_t i = j * NT + tid;
+                    if (i < n) kbr[j] = bj[j] | (hist_rank(hist, bj[j]) << 8);
+                }
+            }
+            __syncthreads();
+            // 3. bucket starts (wave 0 scans nb <= 64 counts)
+            if (wave == 0) {
+                const uint32_t hv = (lane < nb) ? hist[lane] : 0u;
+                uint32_t x = hv;
+#pragma unroll
+                for (int dd = 1; dd < 64; dd <<= 1) {
+                    const uint32_t y = __shfl_up(x, dd, 64);
+                    if (lane >= (uint32_t)dd) x += y;
+                }
+                bst[lane] = x - hv;
+                if (lane == 63) bst[64] = x;
+            }
+            __syncthreads();
+            // 4. place keys by bucket
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t i = j * NT + tid;
+                if (i < n) A[bst[kbr[j] & 0xFFu] + (kbr[j] >> 8)] = kv[j];
+            }
+            __syncthreads();
+            // 5. sub-buckets: bitonic tails, one wave each
+            for (uint32_t b = wave; b < nb; b += NW) {
+                const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
+                if (sz == 0) continue;
+                if (beq[b]) {
+                    const K v = from_key<DT>(sK[beq[b] - 1]);
+                    for (uint32_t i = lane; i < sz; i += 64) out[lo + i] = v;
+                } else if (sz <= (uint32_t)kWaveSortMax) {
+                    const K* As = A + lo;
+                    K* o = out + lo;
+                    wave_sort_any<K>(
+                        sz, [&](uint32_t i) { return As[i]; }, [&](uint32_t i, K v) { o[i] = from_key<DT>(v); });
+                } else if (lane == 0) {
+                    bigs[atomicAdd(&s_nbig, 1u)] = (uint8_t)b;
+                }
+            }
+            __syncthreads();
+            // 6. oversized sub-buckets: whole work-group in LDS
+            const uint32_t nbig = s_nbig;
+            for (uint32_t q = 0; q < nbig; ++q) {
+                const uint32_t b = bigs[q];
+                const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
+                lds_bitonic<K>(A + lo, sz, tid, NT);
+                for (uint32_t i = tid; i < sz; i += NT) out[lo + i] = from_key<DT>(A[lo + i]);
+                __syncthreads();
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -774,8 +1148,11 @@ hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid
 template <int DT>
 static hipError_t leaf_t(const LeafArgs& a, const Grid& g, hipStream_t st) {
     using K = typename KeyOf<DT>::K;
-    const size_t smem = (size_t)a.leaf_max * sizeof(K);
-    hipLaunchKernelGGL(k_leaf<DT>, dim3(g.leaf), dim3(kLeafWG), smem, st, a);
+    hipLaunchKernelGGL((k_leaf<DT, false>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
+    if (a.leaf_max > (uint32_t)kWaveSortMax || a.is_last) {
+        const size_t smem = (size_t)std::max<uint32_t>(a.leaf_max, kWaveSortMax) * sizeof(K);
+        hipLaunchKernelGGL((k_leaf<DT, true>), dim3(g.leaf), dim3(kLeafBigWG), smem, st, a);
+    }
     return hipGetLastError();
 }
 

@@ -76,6 +76,8 @@ _SIGS = {
     "gqs_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
     "gqs_kernel_times": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double),
                                         ctypes.c_int]),
+    "gqs_kernel_trace": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float),
+                                         ctypes.c_int]),
     "gqs_device_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_size_t]),
     "gqs_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _P, ctypes.c_size_t]),
@@ -222,6 +224,14 @@ class Context:
         ms = (ctypes.c_double * n)()
         _check(self._lib.gqs_kernel_times(self._h, launches, ms, n), self)
         return {k: (launches[i], ms[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+    def kernel_trace(self):
+        """Per-launch (kind, ms) list of the last profiled sort (after kernel_times())."""
+        n = self._lib.gqs_kernel_trace(self._h, None, None, 0)
+        kinds = (ctypes.c_int * max(n, 1))()
+        ms = (ctypes.c_float * max(n, 1))()
+        self._lib.gqs_kernel_trace(self._h, kinds, ms, n)
+        return [(KERNEL_KINDS[kinds[i]], ms[i]) for i in range(n)]
 
     def device_info(self):
         cus, lds, wave = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -57,7 +57,7 @@ typedef struct gqs_ctx gqs_ctx;
  * run-time parameters chosen by the autotuner). */
 typedef struct gqs_params {
     int32_t mode;          /* gqs_mode */
-    uint32_t leaf_max;     /* largest bucket sorted in LDS (QUICKSORT_BLOCK_SIZE role); 16..16384 (u32/f32), ..8192 (f64) */
+    uint32_t leaf_max;     /* largest bucket sorted in LDS (QUICKSORT_BLOCK_SIZE role); 16..8192 (u32/f32); f64 uses at most 4096 */
     uint32_t leaf_target;  /* multipivot: bucket size the fan-out aims for */
     uint32_t oversample;   /* multipivot: sample keys per bucket */
     uint32_t max_fanout;   /* multipivot: largest fan-out, power of two in [2, 256] */
@@ -108,6 +108,9 @@ gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
  * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init, 8 fill. */
 gqs_status gqs_set_profiling(gqs_ctx* ctx, int on);
 gqs_status gqs_kernel_times(gqs_ctx* ctx, uint32_t* launches, double* ms, int nkinds);
+/* Per-launch (kind, ms) records collected by the last gqs_kernel_times();
+ * copies up to cap records and returns how many there are (-1 on bad args). */
+int gqs_kernel_trace(gqs_ctx* ctx, int* kinds, float* ms, int cap);
 
 /* Device properties used for launch shapes (CUs, LDS per work-group, name). */
 gqs_status gqs_device_info(gqs_ctx* ctx, int* cus, int* lds_per_wg, int* wave_size, char* name, size_t name_len);

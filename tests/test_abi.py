@@ -16,7 +16,7 @@ def declared_symbols():
     for name in os.listdir(os.path.join(ROOT, "include")):
         if name.endswith(".h"):
             text = open(os.path.join(ROOT, "include", name)).read()
-            syms |= set(re.findall(r"^\s*(?:gqs_status|const char\*|void)\s+(gqs_\w+)\s*\(", text, re.M))
+            syms |= set(re.findall(r"^\s*(?:gqs_status|const char\*|void|int)\s+(gqs_\w+)\s*\(", text, re.M))
     return syms
 
 
