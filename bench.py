@@ -38,7 +38,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb):
+def measured_traffic(kernel, workload):
+    """HBM bytes per sort of `kernel` on `workload` from the committed rocprofv3
+    table (profiles/traffic.txt), or None when that workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and not line.startswith("#") and f[0] == f"k_{kernel}" and f[1] == workload:
+            return float(f[2])
+    return None
+
+
+def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb, workload):
     """Per-kernel event timing of one sort; returns the dominant kernel's roofline."""
     gqsort.load().gqs_memcpy_async(ctx.handle, d.ptr, pristine.ptr, nbytes, stream.handle)
     ctx.set_profiling(True)
@@ -55,9 +68,11 @@ def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb):
     alg_bytes_per_launch = ALG_BYTES_PER_KEY[dom] * kb * keys[dom] / launches
     avg_s = ms / launches / 1e3
     achieved = alg_bytes_per_launch / avg_s / 1e9
+    per_sort = measured_traffic(dom, workload)
+    traffic = None if per_sort is None else int(per_sort / launches)
     return {
         "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-        "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
+        "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": traffic,
         "alg_bytes_per_launch": int(alg_bytes_per_launch), "avg_launch_ms": round(ms / launches, 4),
         "launches": launches,
     }, {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in times.items() if v[0]}, level_keys
@@ -150,7 +165,8 @@ def main():
             sys.exit(1)
 
     log("per-kernel profile pass")
-    roof, kernels, level_keys = kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb)
+    roof, kernels, level_keys = kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb,
+                                                f"{args.dtype}:{n}:{args.gen}")
 
     cpu = par = None
     if not args.no_cpu:

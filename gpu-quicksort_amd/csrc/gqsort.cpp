@@ -58,6 +58,7 @@ void Workspace::release() {
     hipFree(plans);
     hipFree(spl);
     hipFree(meta);
+    hipFree(lut);
     hipFree(mat);
     hipFree(part);
     hipFree(bkt);
@@ -110,6 +111,7 @@ gqs_status Ctx::ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params&
     if (!e) e = dalloc(&ws.plans, c.seg);
     if (!e) e = hipMalloc(&ws.spl, std::max<size_t>(1, c.seg * kSplStride * 8));
     if (!e) e = dalloc(&ws.meta, c.seg * kSplStride);
+    if (!e) e = dalloc(&ws.lut, c.seg * kLutCells);
     if (!e) e = dalloc(&ws.mat, c.mat);
     if (!e) e = dalloc(&ws.part, c.part);
     if (!e) e = dalloc(&ws.bkt, c.bkt);
@@ -206,6 +208,7 @@ gqs_status Ctx::run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* of
     a.plans = ws.plans;
     a.spl = ws.spl;
     a.meta = ws.meta;
+    a.lut = ws.lut;
     a.mat = ws.mat;
     a.part = ws.part;
     a.bkt = ws.bkt;

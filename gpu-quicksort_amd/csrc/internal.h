@@ -22,6 +22,7 @@ struct Workspace {
     SegPlan* plans = nullptr;
     void* spl = nullptr;
     uint16_t* meta = nullptr;
+    uint8_t* lut = nullptr;
     uint32_t* mat = nullptr;
     uint32_t* part = nullptr;
     BucketRec* bkt = nullptr;

@@ -27,6 +27,7 @@ struct LevelArgs {
     SegPlan* plans;          // per segment
     void* spl;               // splitters, kSplStride per segment (K)
     uint16_t* meta;          // bucket-id table, kSplStride per segment
+    uint8_t* lut;            // splitter lookup table, kLutCells per segment
     uint32_t* mat;           // count matrix (scanned in place)
     uint32_t* part;          // scan chunk partials
     BucketRec* bkt;          // per bucket of this level

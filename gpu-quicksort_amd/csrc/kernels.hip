@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
     __shared__ K samp[kSampleMax];
     __shared__ K cand[kSplStride];
     __shared__ uint32_t wsum[kPartWG / 64];
-    __shared__ K s_pivot;
+    __shared__ uint16_t lutw[kLutCells + 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t nseg = a.ctl->nseg;
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
@@ -323,7 +323,6 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
                 K m;
                 if (x1 < x2) m = (x2 < x3) ? x2 : ((x1 < x3) ? x3 : x1);
                 else m = (x1 < x3) ? x1 : ((x2 < x3) ? x3 : x2);
-                s_pivot = m;
                 cand[0] = m;
                 cand[1] = m;  // a repeated candidate: the pivot gets its equality bucket
             }
@@ -373,7 +372,47 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             spl[i] = key_max<K>();
             meta[i] = (uint16_t)((i == m) ? (m + neq) : (nb - 1));
         }
+        // Lookup table over [u0, u_{m-1}] (the unique splitters, staged in samp).
+        K* su = samp;
+        if (first) su[uidx] = c;
+        __syncthreads();
+        const K lo = su[0], range = su[m - 1] - su[0];
+        uint32_t bitlen = 0;
+        while (bitlen < 8 * sizeof(K) && (range >> bitlen) != 0) ++bitlen;
+        const uint32_t shift = bitlen > (uint32_t)kLutBits ? bitlen - kLutBits : 0u;
+        for (uint32_t cidx = tid; cidx < (uint32_t)kLutCells; cidx += kPartWG) {
+            const K off = (K)cidx << shift;
+            uint32_t cnt = m;
+            if (off <= range) {
+                const K v = lo + off;
+                uint32_t l = 0, h = m;  // #{u < v}
+                while (l < h) {
+                    const uint32_t mid = (l + h) >> 1;
+                    if (su[mid] < v) l = mid + 1;
+                    else h = mid;
+                }
+                cnt = l;
+            }
+            lutw[cidx] = (uint16_t)cnt;
+        }
+        if (tid == 0) lutw[kLutCells] = (uint16_t)m;
+        __syncthreads();
+        uint32_t wmax = 1;
+        for (uint32_t cidx = tid; cidx < (uint32_t)kLutCells; cidx += kPartWG) {
+            wmax = max(wmax, (uint32_t)lutw[cidx + 1] - lutw[cidx] + 1u);
+            a.lut[(size_t)s * kLutCells + cidx] = (uint8_t)lutw[cidx];
+        }
+        // block max of the window sizes
+        uint32_t wm = wmax;
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) wm = max(wm, (uint32_t)__shfl_xor(wm, dd, 64));
+        if (lane_id() == 0) wsum[tid >> 6] = wm;
+        __syncthreads();
         if (tid == 0) {
+            uint32_t w = 1;
+            for (int i = 0; i < kPartWG / 64; ++i) w = max(w, wsum[i]);
+            uint32_t ld = 0;
+            while ((1u << ld) < w) ++ld;
             uint32_t depth = 0;
             while (((1u << depth) - 1u) < m) ++depth;
             SegPlan p;
@@ -383,6 +422,9 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             p.nbuckets = nb;
             p.bucket_base = 0;
             p.depth = depth;
+            p.lut_shift = shift;
+            p.lut_depth = ld;
+            p.lut_lo = (uint64_t)lo;
             a.plans[s] = p;
         }
         __syncthreads();
@@ -443,7 +485,8 @@ __global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
 // ------------------------------------------------------------ tile helpers
 template <class K>
 struct TileLds {
-    K sK[kSplStride];
+    K sK[2 * kSplStride];  // splitters, padded with the maximum key
+    uint8_t sLut[kLutCells];
     uint16_t sMeta[kSplStride];
     uint32_t hist[kSplStride];
     uint32_t seg;
@@ -451,12 +494,57 @@ struct TileLds {
 };
 
 template <class K>
-__device__ __forceinline__ void load_splitters(const LevelArgs& a, uint32_t s, K* sK, uint16_t* sMeta) {
+__device__ __forceinline__ void load_splitters(const LevelArgs& a, uint32_t s, TileLds<K>& L) {
     const K* spl = (const K*)a.spl + (size_t)s * kSplStride;
     const uint16_t* meta = a.meta + (size_t)s * kSplStride;
     for (uint32_t i = threadIdx.x; i < (uint32_t)kSplStride; i += blockDim.x) {
-        sK[i] = spl[i];
-        sMeta[i] = meta[i];
+        L.sK[i] = spl[i];
+        L.sK[i + kSplStride] = key_max<K>();
+        L.sMeta[i] = meta[i];
+    }
+    const uint32_t* lut = reinterpret_cast<const uint32_t*>(a.lut + (size_t)s * kLutCells);
+    uint32_t* dl = reinterpret_cast<uint32_t*>(L.sLut);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kLutCells / 4; i += blockDim.x) dl[i] = lut[i];
+}
+
+// Classification through the lookup table: the table gives the first
+// candidate for #{splitters < key}, D more binary-search steps finish it.
+template <class K, int D, int N>
+__device__ __forceinline__ void classify_lut_d(const K (&key)[N], uint32_t (&b)[N], const TileLds<K>& L, K lo,
+                                               uint32_t shift) {
+    uint32_t i[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const K off = key[j] - lo;
+        const K cell = (key[j] < lo) ? K(0) : min(off >> shift, (K)(kLutCells - 1));
+        i[j] = L.sLut[(uint32_t)cell];
+    }
+#pragma unroll
+    for (int st = D - 1; st >= 0; --st) {
+        const uint32_t step = 1u << st;
+#pragma unroll
+        for (int j = 0; j < N; ++j) i[j] += (L.sK[i[j] + step - 1] < key[j]) ? step : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint32_t meta = L.sMeta[i[j]];
+        b[j] = (meta & 0xFFu) + (((meta >> 8) & 1u) & (uint32_t)(key[j] == L.sK[i[j]]));
+    }
+}
+template <class K, int N>
+__device__ __forceinline__ void classify_lut(const K (&key)[N], uint32_t (&b)[N], const TileLds<K>& L,
+                                             const SegPlan& p) {
+    const K lo = (K)p.lut_lo;
+    switch (p.lut_depth) {
+        case 0: classify_lut_d<K, 0, N>(key, b, L, lo, p.lut_shift); break;
+        case 1: classify_lut_d<K, 1, N>(key, b, L, lo, p.lut_shift); break;
+        case 2: classify_lut_d<K, 2, N>(key, b, L, lo, p.lut_shift); break;
+        case 3: classify_lut_d<K, 3, N>(key, b, L, lo, p.lut_shift); break;
+        case 4: classify_lut_d<K, 4, N>(key, b, L, lo, p.lut_shift); break;
+        case 5: classify_lut_d<K, 5, N>(key, b, L, lo, p.lut_shift); break;
+        case 6: classify_lut_d<K, 6, N>(key, b, L, lo, p.lut_shift); break;
+        case 7: classify_lut_d<K, 7, N>(key, b, L, lo, p.lut_shift); break;
+        default: classify_lut_d<K, 8, N>(key, b, L, lo, p.lut_shift); break;
     }
 }
 
@@ -491,7 +579,7 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
         __syncthreads();
         const uint32_t s = L.seg;
         if (s != cur) {
-            load_splitters<K>(a, s, L.sK, L.sMeta);
+            load_splitters<K>(a, s, L);
             cur = s;
         }
         L.hist[tid] = 0;
@@ -504,20 +592,30 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
         const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
         const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
         const K* base = src + seg.start;
+        K keys[TS::kKeysPerThread];
+#pragma unroll
+        for (int j = 0; j < TS::kKeysPerThread; ++j) {
+            const uint32_t idx = lo + j * kPartWG + tid;
+            keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+        }
         for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
-            K keys[TS::kKeysPerThread];
+            // prefetch the next sub-tile while this one is classified
+            K nxt[TS::kKeysPerThread];
+            const uint32_t nsub = sub + TS::kSubKeys;
 #pragma unroll
             for (int j = 0; j < TS::kKeysPerThread; ++j) {
-                const uint32_t idx = sub + j * kPartWG + tid;
-                keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+                const uint32_t idx = nsub + j * kPartWG + tid;
+                nxt[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
             }
             uint32_t bk[TS::kKeysPerThread];
-            classify_n<K, TS::kKeysPerThread>(keys, bk, L.sK, L.sMeta, p.depth);
+            classify_lut<K, TS::kKeysPerThread>(keys, bk, L, p);
 #pragma unroll
             for (int j = 0; j < TS::kKeysPerThread; ++j) {
                 const uint32_t idx = sub + j * kPartWG + tid;
                 if (idx < hi) hist_add(L.hist, bk[j]);
             }
+#pragma unroll
+            for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = nxt[j];
         }
         __syncthreads();
         for (uint32_t b = tid; b < p.nbuckets; b += kPartWG) a.mat[p.mat_base + b * p.ntiles + tl] = L.hist[b];
@@ -645,13 +743,15 @@ template <int DT>
 __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
     using TS = TileShape<K>;
+    constexpr int KPT = TS::kKeysPerThread;
     __shared__ TileLds<K> L;
     __shared__ uint32_t lstart[kSplStride];
-    __shared__ uint32_t cursor[kSplStride];
+    __shared__ uint32_t delta[kSplStride];  // output offset - staging offset, per bucket
     __shared__ uint8_t direct[kSplStride];
     __shared__ K stage[TS::kSubKeys];
     __shared__ uint8_t sbk[TS::kSubKeys];
     __shared__ uint32_t wsum[kPartWG / 64];
+    __shared__ uint32_t s_anydirect;
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = a.ctl->nseg;
@@ -673,53 +773,67 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
                 seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
             }
             L.seg = seg_i;
+            s_anydirect = 0;
         }
         __syncthreads();
         const uint32_t s = L.seg;
         if (s != cur) {
-            load_splitters<K>(a, s, L.sK, L.sMeta);
+            load_splitters<K>(a, s, L);
             cur = s;
         }
         const SegRec seg = a.segs[s];
         const SegPlan p = a.plans[s];
         const uint32_t tl = t - p.tile_begin;
+        // Thread b owns bucket b: its running output offset inside the segment.
+        uint32_t my_cursor = 0;
         if (tid < p.nbuckets) {
             const uint32_t m0 = a.mat[p.mat_base];
-            cursor[tid] = a.mat[p.mat_base + tid * p.ntiles + tl] - m0;
-            direct[tid] = (a.bkt[p.bucket_base + tid].size & kBucketDirect) ? 1 : 0;
+            my_cursor = a.mat[p.mat_base + tid * p.ntiles + tl] - m0;
+            const bool dir = (a.bkt[p.bucket_base + tid].size & kBucketDirect) != 0;
+            direct[tid] = dir ? 1 : 0;
+            if (dir) s_anydirect = 1;
         }
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
-        K* dst = (seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn;
+        K* dst = ((seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn) + seg.start;
         const bool raw = seg.flags & SEG_RAW;
         const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
         const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
         const K* base = src + seg.start;
+        K keys[KPT];
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            const uint32_t idx = lo + j * kPartWG + tid;
+            keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+        }
         for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
             L.hist[tid] = 0;
             __syncthreads();
-            K keys[TS::kKeysPerThread];
-            uint32_t bk[TS::kKeysPerThread];
-            uint32_t rk[TS::kKeysPerThread];
+            uint32_t bk[KPT];
+            uint32_t rk[KPT];
+            classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) {
-                const uint32_t idx = sub + j * kPartWG + tid;
-                keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
-            }
-            classify_n<K, TS::kKeysPerThread>(keys, bk, L.sK, L.sMeta, p.depth);
-#pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+            for (int j = 0; j < KPT; ++j) {
                 const uint32_t idx = sub + j * kPartWG + tid;
                 rk[j] = 0;
                 if (idx < hi) rk[j] = hist_rank(L.hist, bk[j]);
+            }
+            // prefetch the next sub-tile while this one is staged and stored
+            K nxt[KPT];
+            const uint32_t nsub = sub + TS::kSubKeys;
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t idx = nsub + j * kPartWG + tid;
+                nxt[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
             }
             __syncthreads();
             uint32_t cnt;
             const uint32_t hv = (tid < p.nbuckets) ? L.hist[tid] : 0u;
             const uint32_t ls = block_excl_scan<kPartWG>(hv, wsum, &cnt);
             lstart[tid] = ls;
+            delta[tid] = my_cursor - ls;
             __syncthreads();
 #pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) {
+            for (int j = 0; j < KPT; ++j) {
                 const uint32_t idx = sub + j * kPartWG + tid;
                 if (idx < hi) {
                     const uint32_t pos = lstart[bk[j]] + rk[j];
@@ -728,16 +842,19 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
                 }
             }
             __syncthreads();
-            for (uint32_t i = tid; i < cnt; i += kPartWG) {
-                const uint32_t b = sbk[i];
-                if (direct[b]) continue;  // equality bucket: filled by k_fill
-                const uint64_t pos = seg.start + cursor[b] + (i - lstart[b]);
-                dst[pos] = stage[i];
+            if (s_anydirect) {
+                for (uint32_t i = tid; i < cnt; i += kPartWG) {
+                    const uint32_t b = sbk[i];
+                    if (!direct[b]) dst[delta[b] + i] = stage[i];  // equality buckets: k_fill
+                }
+            } else {
+                for (uint32_t i = tid; i < cnt; i += kPartWG) dst[delta[sbk[i]] + i] = stage[i];
             }
+            my_cursor += hv;
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
             __syncthreads();
-            if (tid < p.nbuckets) cursor[tid] += hv;
         }
-        __syncthreads();
     }
 }
 
@@ -757,6 +874,7 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
 constexpr int kWaveSortMax = 512;
 constexpr int kLeafSplit = 64;  // largest local fan-out
 constexpr int kLeafBigWG = 512;
+constexpr int kLeafTail = 256;  // sub-bucket size the local fan-out aims for
 
 template <class K>
 __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
@@ -855,6 +973,94 @@ __device__ __forceinline__ void wave_bitonic(K (&v)[E]) {
     }
 }
 
+// NB independent problems of 64*E keys each, sorted together by one wave
+// (problem q in v[q][*]); the problems' steps interleave, so the wave has NB*E
+// independent compare-exchange chains.
+template <class K, int E, int NB>
+__device__ __forceinline__ void wave_bitonic_nb(K (&v)[NB][E]) {
+    constexpr int LE = Log2<E>::v;
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int m = 1; m <= 6 + LE; ++m) {
+        const int k = 1 << m;
+        if (k <= E) {
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const int p = r ^ (k - 1);
+                    if (p > r) {
+                        const K lo = kmin(v[q][r], v[q][p]), hi = kmax(v[q][r], v[q][p]);
+                        v[q][r] = lo;
+                        v[q][p] = hi;
+                    }
+                }
+        } else {
+            const uint32_t lm = (uint32_t)(k - 1) >> LE;
+            const bool upper = (lane & ((uint32_t)(k >> 1) >> LE)) != 0;
+            K t[NB][E];
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+#pragma unroll
+                for (int r = 0; r < E; ++r) t[q][r] = shfl_xor_k(v[q][E - 1 - r], lm);
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+#pragma unroll
+                for (int r = 0; r < E; ++r) v[q][r] = upper ? kmax(v[q][r], t[q][r]) : kmin(v[q][r], t[q][r]);
+        }
+#pragma unroll
+        for (int j = k >> 2; j >= 1; j >>= 1) {
+            if (j < E) {
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+#pragma unroll
+                    for (int r = 0; r < E; ++r) {
+                        if (!(r & j)) {
+                            const K lo = kmin(v[q][r], v[q][r + j]), hi = kmax(v[q][r], v[q][r + j]);
+                            v[q][r] = lo;
+                            v[q][r + j] = hi;
+                        }
+                    }
+            } else {
+                const uint32_t lm = (uint32_t)j >> LE;
+                const bool upper = (lane & lm) != 0;
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+#pragma unroll
+                    for (int r = 0; r < E; ++r) {
+                        const K t = shfl_xor_k(v[q][r], lm);
+                        v[q][r] = upper ? kmax(v[q][r], t) : kmin(v[q][r], t);
+                    }
+            }
+        }
+    }
+}
+
+// One wave sorts up to NB sub-buckets of at most 64*E keys each from LDS:
+// sub-bucket q is A[lo[q], lo[q]+sz[q]) and goes to out[lo[q] ...].
+template <int DT, int E, int NB>
+__device__ __forceinline__ void wave_sort_batch(const typename KeyOf<DT>::K* A, typename KeyOf<DT>::K* out,
+                                                const uint32_t (&lo)[NB], const uint32_t (&sz)[NB]) {
+    using K = typename KeyOf<DT>::K;
+    const uint32_t lane = lane_id();
+    K v[NB][E];
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            v[q][r] = (i < sz[q]) ? A[lo[q] + i] : key_max<K>();
+        }
+    wave_bitonic_nb<K, E, NB>(v);
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            if (i < sz[q]) out[lo[q] + i] = from_key<DT>(v[q][r]);
+        }
+}
+
 // One wave sorts n <= 64*E keys read through `ld(i)` and stores key i with `st(i, key)`.
 template <class K, int E, class LD, class ST>
 __device__ __forceinline__ void wave_sort_n(uint32_t n, LD ld, ST st) {
@@ -908,6 +1114,8 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
     __shared__ uint32_t bst[kLeafSplit + 1];
     __shared__ uint32_t s_nb, s_depth, s_nbig;
     __shared__ uint8_t bigs[kLeafSplit];
+    __shared__ uint8_t tl_list[4][kLeafSplit];  // sub-buckets per size class
+    __shared__ uint32_t tl_n[4];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
@@ -987,7 +1195,7 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
             }
             // 2. splitters from a stratified sample, sorted by wave 0
             uint32_t k = 2;
-            while (k < kLeafSplit && k * 256 < n) k <<= 1;
+            while (k < kLeafSplit && k * kLeafTail < n) k <<= 1;
             const uint32_t S = min((uint32_t)kWaveSortMax, 16u * k);
             if (tid <= (uint32_t)kLeafSplit) hist[tid] = 0;
             if (tid < S) {
@@ -1074,20 +1282,64 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
                 if (i < n) A[bst[kbr[j] & 0xFFu] + (kbr[j] >> 8)] = kv[j];
             }
             __syncthreads();
-            // 5. sub-buckets: bitonic tails, one wave each
+            // 5. sub-buckets.  Wave 0 sorts them into size classes (one lane
+            //    per sub-bucket, nb <= 64); equality sub-buckets are filled;
+            //    sub-buckets of <= 64 keys are sorted four per wave at once,
+            //    <= 128 two per wave, <= 512 one per wave.
+            if (wave == 0) {
+                const uint32_t b = lane;
+                const uint32_t sz = (b < nb) ? bst[b + 1] - bst[b] : 0u;
+                const bool valid = b < nb && sz > 0 && !beq[b];
+                const uint32_t cls = !valid ? 4u : (sz <= 64 ? 0u : (sz <= 128 ? 1u : (sz <= (uint32_t)kWaveSortMax ? 2u : 3u)));
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) {
+                    const uint64_t mk = __ballot(cls == c);
+                    if (cls == c) tl_list[c][mbcnt(mk)] = (uint8_t)b;
+                    if (lane == 0) tl_n[c] = (uint32_t)__popcll(mk);
+                }
+            }
+            __syncthreads();
             for (uint32_t b = wave; b < nb; b += NW) {
+                if (!beq[b]) continue;
                 const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
-                if (sz == 0) continue;
-                if (beq[b]) {
-                    const K v = from_key<DT>(sK[beq[b] - 1]);
-                    for (uint32_t i = lane; i < sz; i += 64) out[lo + i] = v;
-                } else if (sz <= (uint32_t)kWaveSortMax) {
+                const K v = from_key<DT>(sK[beq[b] - 1]);
+                for (uint32_t i = lane; i < sz; i += 64) out[lo + i] = v;
+            }
+            {
+                const uint32_t nA = tl_n[0];
+                for (uint32_t g = wave * 4; g < nA; g += NW * 4) {
+                    uint32_t lo4[4], sz4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t b = (g + q < nA) ? tl_list[0][g + q] : 0u;
+                        lo4[q] = bst[b];
+                        sz4[q] = (g + q < nA) ? bst[b + 1] - bst[b] : 0u;
+                    }
+                    wave_sort_batch<DT, 1, 4>(A, out, lo4, sz4);
+                }
+                const uint32_t nB = tl_n[1];
+                for (uint32_t g = wave * 2; g < nB; g += NW * 2) {
+                    uint32_t lo2[2], sz2[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const uint32_t b = (g + q < nB) ? tl_list[1][g + q] : 0u;
+                        lo2[q] = bst[b];
+                        sz2[q] = (g + q < nB) ? bst[b + 1] - bst[b] : 0u;
+                    }
+                    wave_sort_batch<DT, 2, 2>(A, out, lo2, sz2);
+                }
+                const uint32_t nC = tl_n[2];
+                for (uint32_t g = wave; g < nC; g += NW) {
+                    const uint32_t b = tl_list[2][g];
+                    const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
                     const K* As = A + lo;
                     K* o = out + lo;
                     wave_sort_any<K>(
                         sz, [&](uint32_t i) { return As[i]; }, [&](uint32_t i, K v) { o[i] = from_key<DT>(v); });
-                } else if (lane == 0) {
-                    bigs[atomicAdd(&s_nbig, 1u)] = (uint8_t)b;
+                }
+                if (tid == 0) {
+                    s_nbig = tl_n[3];
+                    for (uint32_t q = 0; q < tl_n[3]; ++q) bigs[q] = tl_list[3][q];
                 }
             }
             __syncthreads();

@@ -45,7 +45,15 @@ struct SegPlan {
     uint32_t nbuckets;     // buckets of this segment (m + 1 + #equality buckets)
     uint32_t bucket_base;  // first bucket of this segment in the level's bucket space
     uint32_t depth;        // binary-search depth: 2^depth - 1 >= number of splitters
+    uint32_t lut_shift;    // lookup table: cell = (key - lut_lo) >> lut_shift
+    uint32_t lut_depth;    // search steps after the lookup (max over cells)
+    uint64_t lut_lo;       // lowest splitter
 };
+
+// Splitter lookup table: kLutCells cells over [u0, u_last]; cell c holds
+// #{splitters < u0 + (c << shift)}, the first candidate of the binary search.
+constexpr int kLutBits = 10;
+constexpr int kLutCells = 1 << kLutBits;
 
 // Per-level control block in HBM.
 struct LevelCtl {
