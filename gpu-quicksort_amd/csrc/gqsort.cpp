@@ -88,7 +88,7 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     return c;
 }
 
-gqs_status Ctx::ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
+gqs_status Ctx::ensure_workspace(Workspace& ws, uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     const Caps need = caps_for(n, nseg_in, p, dtype);
     const size_t kb = key_bytes(dtype);
     if (ws.ctl && need.seg <= ws.cap.seg && need.mat <= ws.cap.mat && need.bkt <= ws.cap.bkt &&
@@ -139,7 +139,7 @@ Grid Ctx::grid() const {
 }
 
 void Ctx::prof_begin(int kind, hipStream_t st) {
-    if (!profiling) return;
+    if (!profiling || capturing_now) return;
     ProfRec r;
     r.kind = kind;
     hipEventCreate(&r.a);
@@ -148,7 +148,7 @@ void Ctx::prof_begin(int kind, hipStream_t st) {
     prof.push_back(r);
 }
 void Ctx::prof_end(hipStream_t st) {
-    if (!profiling || prof.empty()) return;
+    if (!profiling || capturing_now || prof.empty()) return;
     hipEventRecord(prof.back().b, st);
 }
 
@@ -166,15 +166,18 @@ uint32_t levels_needed(uint64_t n, const gqs_params& p) {
 
 // The whole sort as one stream-ordered chain.  Either a single key range
 // [0, n) of d (offsets == nullptr) or nseg arrays delimited by offsets.
-gqs_status Ctx::run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets, uint32_t nseg_in,
-                    hipStream_t st) {
+gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
+                    uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing) {
     gqs_params p = params;
+    if (capturing) p.host_driven = 0;
     // a work-group leaf holds 32 (u32/f32) or 16 (f64) keys per thread
     const uint32_t cap = (dtype == DT_F64) ? 4096u : 8192u;
     p.leaf_max = std::min(p.leaf_max, cap);
-    if (gqs_status s = ensure_workspace(n, nseg_in, p, dtype)) return s;
-    last_stream = st;
-    if (profiling) {
+    if (gqs_status s = ensure_workspace(ws, n, nseg_in, p, dtype)) return s;
+    if (!capturing) last_stream = st;
+    capturing_now = capturing;
+    const bool prof_on = profiling && !capturing;
+    if (prof_on) {
         for (auto& r : prof) {
             hipEventDestroy(r.a);
             hipEventDestroy(r.b);
@@ -187,8 +190,8 @@ gqs_status Ctx::run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* of
         L = levels_needed(n, p);
         prof_begin(7, st);
         if (fail(launch_init(ws.ctl, kMaxLevels, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
-        if (fail(launch_seed_segments(offsets, nseg_in, ws.segs[0], &ws.ctl[0], ws.leaves, &ws.ctl[0].nleaf_list,
-                                      p.leaf_max, st),
+        if (fail(launch_seed_segments(offsets, nseg_in, nseg_dev, ws.segs[0], &ws.ctl[0], ws.leaves,
+                                      &ws.ctl[0].nleaf_list, p.leaf_max, st),
                  "seed"))
             return GQS_EDEVICE;
         prof_end(st);
@@ -388,7 +391,7 @@ gqs_status gqs_sort_device(gqs_ctx* c, int dtype, void* d, void* dn, size_t n, v
     if (!d || !dn) return c->set_error(GQS_EINVAL, "gqs_sort_device: null buffer");
     if (n >= 0xFFFFFFFFull) return c->set_error(GQS_EINVAL, "gqs_sort_device: n must be < 2^32");
     if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
-    return c->run(dtype, d, dn, n, nullptr, 0, (hipStream_t)stream);
+    return c->run(c->ws, dtype, d, dn, n, nullptr, 0, nullptr, (hipStream_t)stream, false);
 }
 
 gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
@@ -400,7 +403,7 @@ gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const ui
     if (!d || !dn || !offsets) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: null buffer");
     if (nseg >= 0xFFFFFFFFull) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: too many arrays");
     if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
-    return c->run(dtype, d, dn, total_keys, offsets, (uint32_t)nseg, (hipStream_t)stream);
+    return c->run(c->ws, dtype, d, dn, total_keys, offsets, (uint32_t)nseg, nullptr, (hipStream_t)stream, false);
 }
 
 gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
@@ -430,6 +433,94 @@ gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
     if (s) return s;
     std::lock_guard<std::mutex> lk(c->mu);
     if (c->fail(hipMemcpy(d, c->host_d, bytes, hipMemcpyDeviceToHost), "D2H")) return GQS_EDEVICE;
+    return GQS_OK;
+}
+
+gqs_status gqs_sort_each(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* host_offsets, size_t nseg,
+                         void* stream) {
+    if (!c || !host_offsets) return GQS_EINVAL;
+    const size_t kb = key_bytes(dtype);
+    for (size_t i = 0; i < nseg; ++i) {
+        const uint64_t lo = host_offsets[i], hi = host_offsets[i + 1];
+        if (hi < lo) return c->set_error(GQS_EINVAL, "gqs_sort_each: offsets must be non-decreasing");
+        gqs_status s = gqs_sort_device(c, dtype, static_cast<char*>(d) + lo * kb, static_cast<char*>(dn) + lo * kb,
+                                       hi - lo, stream);
+        if (s) return s;
+    }
+    return GQS_OK;
+}
+
+// ---- captured segmented sort ------------------------------------------------
+struct gqs_graph {
+    gqs_ctx* ctx = nullptr;
+    gqs::Workspace ws;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+gqs_status gqs_graph_create_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* offsets,
+                                      const uint32_t* nseg_dev, size_t max_nseg, size_t max_total_keys,
+                                      gqs_graph** out) {
+    if (!c || !out) return GQS_EINVAL;
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_graph_create_segmented: bad dtype");
+    if (!d || !dn || !offsets || !nseg_dev || max_nseg == 0 || max_total_keys < 2 || max_nseg >= 0xFFFFFFFFull)
+        return c->set_error(GQS_EINVAL, "gqs_graph_create_segmented: bad argument");
+    if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+    auto* g = new (std::nothrow) gqs_graph();
+    if (!g) return GQS_ENOMEM;
+    g->ctx = c;
+    gqs_params p = c->params;
+    p.leaf_max = std::min(p.leaf_max, dtype == DT_F64 ? 4096u : 8192u);
+    if (gqs_status s = c->ensure_workspace(g->ws, max_total_keys, (uint32_t)max_nseg, p, dtype)) {
+        delete g;
+        return s;
+    }
+    hipStream_t cs = nullptr;
+    if (c->fail(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream")) {
+        g->ws.release();
+        delete g;
+        return GQS_EDEVICE;
+    }
+    gqs_status st = GQS_OK;
+    if (c->fail(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture")) {
+        st = GQS_EDEVICE;
+    } else {
+        st = c->run(g->ws, dtype, d, dn, max_total_keys, offsets, (uint32_t)max_nseg, nseg_dev, cs, true);
+        c->capturing_now = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (st == GQS_OK && c->fail(e, "end capture")) st = GQS_EDEVICE;
+        g->graph = graph;
+        if (st == GQS_OK && c->fail(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0), "instantiate"))
+            st = GQS_EDEVICE;
+    }
+    hipStreamDestroy(cs);
+    if (st != GQS_OK) {
+        if (g->exec) hipGraphExecDestroy(g->exec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        g->ws.release();
+        delete g;
+        return st;
+    }
+    *out = g;
+    return GQS_OK;
+}
+
+gqs_status gqs_graph_launch(gqs_graph* g, void* stream) {
+    if (!g || !g->exec) return GQS_EINVAL;
+    if (g->ctx->fail(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch")) return GQS_EDEVICE;
+    return GQS_OK;
+}
+
+gqs_status gqs_graph_destroy(gqs_graph* g) {
+    if (!g) return GQS_EINVAL;
+    hipDeviceSynchronize();
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    g->ws.release();
+    delete g;
     return GQS_OK;
 }
 

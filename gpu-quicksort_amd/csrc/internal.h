@@ -73,12 +73,15 @@ struct Ctx {
 
     bool fail(hipError_t e, const char* what);
     gqs_status set_error(gqs_status s, const char* msg);
-    gqs_status ensure_workspace(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
+    bool capturing_now = false;
+    gqs_status ensure_workspace(Workspace& w, uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
     Grid grid() const;
     void prof_begin(int kind, hipStream_t st);
     void prof_end(hipStream_t st);
-    gqs_status run(int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets, uint32_t nseg_in,
-                   hipStream_t st);
+    // Enqueue one sort on `st` using workspace `w`.  nseg_dev (optional) is a
+    // device word holding the number of arrays, read when the chain runs.
+    gqs_status run(Workspace& w, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
+                   uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing);
     gqs_status fetch_stats();
 };
 

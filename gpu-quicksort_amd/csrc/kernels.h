@@ -71,7 +71,8 @@ struct Grid {
 hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n,
                        uint32_t flags, hipStream_t st);
 hipError_t launch_leaf(int dtype, const LeafArgs& a, const Grid& g, hipStream_t st);
-hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, SegRec* segs, LevelCtl* ctl0,
-                                SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max, hipStream_t st);
+hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
+                                LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max,
+                                hipStream_t st);
 
 }  // namespace gqs

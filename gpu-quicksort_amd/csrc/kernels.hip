@@ -283,9 +283,12 @@ __global__ void k_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_c
 }
 
 // Batched sort: one segment or leaf per caller array (offsets[i]..offsets[i+1]).
-__global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, SegRec* segs, LevelCtl* ctl0, SegRec* leaves,
-                       uint32_t* nleaves, uint32_t leaf_max) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg_in; i += gridDim.x * blockDim.x) {
+// nseg_dev, when given, holds the number of arrays (at most nseg_in, the
+// capacity the workspace was sized for) and is read when the kernel runs.
+__global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
+                       LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max) {
+    const uint32_t nseg = nseg_dev ? min(*nseg_dev, nseg_in) : nseg_in;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
         const uint64_t lo = offsets[i], hi = offsets[i + 1];
         const uint64_t len = hi - lo;
         if (len < 2) continue;
@@ -1364,11 +1367,12 @@ hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_c
     return hipGetLastError();
 }
 
-hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, SegRec* segs, LevelCtl* ctl0,
-                                SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max, hipStream_t st) {
+hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
+                                LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max,
+                                hipStream_t st) {
     const uint32_t blocks = std::max(1u, std::min(1024u, (nseg_in + 255) / 256));
-    hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(256), 0, st, offsets, nseg_in, segs, ctl0, leaves, nleaves,
-                       leaf_max);
+    hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(256), 0, st, offsets, nseg_in, nseg_dev, segs, ctl0, leaves,
+                       nleaves, leaf_max);
     return hipGetLastError();
 }
 

@@ -81,6 +81,11 @@ _SIGS = {
     "gqs_device_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_size_t]),
     "gqs_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _P, ctypes.c_size_t]),
+    "gqs_sort_each": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P]),
+    "gqs_graph_create_segmented": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t,
+                                                   ctypes.POINTER(_P)]),
+    "gqs_graph_launch": (ctypes.c_int, [_P, _P]),
+    "gqs_graph_destroy": (ctypes.c_int, [_P]),
     "gqs_malloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "gqs_free": (ctypes.c_int, [_P, _P]),
     "gqs_host_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
@@ -252,6 +257,17 @@ def GPUQSort(ctx, size, d, dn):
     _check(load().gqs_sort(ctx.handle, dt, d.ctypes.data, dn.ctypes.data, size), ctx)
 
 
+def batch_lengths(nseg, lo, hi, seed, total=None):
+    """Array lengths floor(2^U), U uniform on [log2 lo, log2 hi] (seeded); with
+    `total`, rescaled so they sum to about `total` (each kept >= 1)."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log2(lo), np.log2(hi), nseg)
+    lens = np.floor(2.0 ** u)
+    if total is not None:
+        lens = np.maximum(1, np.floor(lens * (total / lens.sum())))
+    return lens.astype(np.uint64)
+
+
 def generate(dtype, gen, seed, n):
     """Synthetic input of one of the benchmark generators (host numpy array)."""
     if isinstance(gen, str):
@@ -259,6 +275,25 @@ def generate(dtype, gen, seed, n):
     a = np.empty(n, dtype=NP_DTYPES[dtype])
     _check(load().gqs_generate(dtype, gen, seed, a.ctypes.data if n else None, n))
     return a
+
+
+class SegmentedGraph:
+    """A captured segmented sort (gqs_graph_create_segmented), replayed with launch()."""
+
+    def __init__(self, ctx, dtype, d, dn, offsets, nseg_dev, max_nseg, max_total_keys):
+        self.ctx = ctx
+        h = _P()
+        _check(ctx._lib.gqs_graph_create_segmented(ctx.handle, dtype, _ptr(d), _ptr(dn), _ptr(offsets),
+                                                    _ptr(nseg_dev), max_nseg, max_total_keys, ctypes.byref(h)), ctx)
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(self.ctx._lib.gqs_graph_launch(self._h, _stream(stream)), self.ctx)
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.gqs_graph_destroy(self._h)
+            self._h = None
 
 
 class DeviceBuffer:

@@ -99,6 +99,24 @@ gqs_status gqs_sort_device(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n,
 gqs_status gqs_sort_segmented(gqs_ctx* ctx, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
                               size_t total_keys, void* stream);
 
+/* The per-array baseline of a segmented sort: one plain gqs_sort_device call
+ * per array (host_offsets: nseg+1 HOST uint64), all enqueued on `stream`. */
+gqs_status gqs_sort_each(gqs_ctx* ctx, int dtype, void* d, void* dn, const uint64_t* host_offsets, size_t nseg,
+                         void* stream);
+
+/* Segmented sort captured once into a hipGraph and replayed: the graph owns
+ * its workspace (sized for max_nseg arrays and max_total_keys keys), reads the
+ * current number of arrays from the device word *nseg_dev and the bounds from
+ * offsets[0..nseg] at every replay, and sorts the arrays of d in place (dn is
+ * scratch).  d, dn, offsets and nseg_dev must stay valid while the graph lives;
+ * refill them (stream-ordered) before each gqs_graph_launch. */
+typedef struct gqs_graph gqs_graph;
+gqs_status gqs_graph_create_segmented(gqs_ctx* ctx, int dtype, void* d, void* dn, const uint64_t* offsets,
+                                      const uint32_t* nseg_dev, size_t max_nseg, size_t max_total_keys,
+                                      gqs_graph** out);
+gqs_status gqs_graph_launch(gqs_graph* g, void* stream);
+gqs_status gqs_graph_destroy(gqs_graph* g);
+
 /* Statistics of the last sort (synchronises the context's last stream). */
 gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
 

@@ -1,0 +1,99 @@
+"""Segmented (batched) sort and its captured hipGraph: every array sorted on
+its own, bit-exact against std::sort of that array."""
+import numpy as np
+import pytest
+
+import gqsort
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(dtype, lens, seed):
+    total = int(lens.sum())
+    gen = "normal" if dtype != gqsort.U32 else "uniform"
+    keys = gqsort.generate(dtype, gen, seed, max(total, 1))[:total]
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    return keys, offs
+
+
+def check_batch(got, keys, offs, case):
+    for i in range(len(offs) - 1):
+        lo, hi = int(offs[i]), int(offs[i + 1])
+        exp = orc.std_sort(keys[lo:hi])
+        bad, first = orc.mismatch(got[lo:hi], exp)
+        assert bad == 0, f"{case}: array {i} [{lo},{hi}) {bad} keys differ, first at {first}"
+
+
+EDGE_LENS = [0, 1, 2, 3, 17, 64, 65, 511, 512, 513, 4095, 8192, 8193, 20000, 100000, 0, 5]
+
+
+@pytest.mark.parametrize("dtype", [gqsort.U32, gqsort.F32, gqsort.F64])
+def test_segmented_edges(gpu_ctx, dtype):
+    lens = np.array(EDGE_LENS, np.uint64)
+    keys, offs = make_batch(dtype, lens, 3)
+    print(f"case segmented dtype={dtype} arrays={len(lens)}", flush=True)
+    d = gqsort.DeviceBuffer.from_array(gpu_ctx, keys)
+    dn = gqsort.DeviceBuffer(gpu_ctx, keys.nbytes)
+    od = gqsort.DeviceBuffer.from_array(gpu_ctx, offs)
+    gpu_ctx.sort_segmented(dtype, d, dn, od, len(lens), keys.size)
+    check_batch(d.download(keys.dtype, keys.size), keys, offs, f"segmented/{dtype}")
+
+
+def test_segmented_log_uniform(gpu_ctx):
+    lens = gqsort.batch_lengths(2000, 16, 1 << 16, seed=4)
+    keys, offs = make_batch(gqsort.U32, lens, 5)
+    print(f"case segmented log-uniform total={keys.size}", flush=True)
+    d = gqsort.DeviceBuffer.from_array(gpu_ctx, keys)
+    dn = gqsort.DeviceBuffer(gpu_ctx, keys.nbytes)
+    od = gqsort.DeviceBuffer.from_array(gpu_ctx, offs)
+    gpu_ctx.sort_segmented(gqsort.U32, d, dn, od, len(lens), keys.size)
+    check_batch(d.download(keys.dtype, keys.size), keys, offs, "log-uniform")
+
+
+def test_sort_each_matches(gpu_ctx):
+    lens = np.array([5, 9000, 300, 70000], np.uint64)
+    keys, offs = make_batch(gqsort.U32, lens, 6)
+    d = gqsort.DeviceBuffer.from_array(gpu_ctx, keys)
+    dn = gqsort.DeviceBuffer(gpu_ctx, keys.nbytes)
+    gqsort._check(gqsort.load().gqs_sort_each(gpu_ctx.handle, gqsort.U32, d.ptr, dn.ptr, offs.ctypes.data,
+                                               len(lens), None), gpu_ctx)
+    check_batch(d.download(keys.dtype, keys.size), keys, offs, "sort_each")
+
+
+def test_graph_replays(gpu_ctx):
+    """Capture once, replay four times with new arrays (and a smaller count on
+    two replays), with eager sorts of the same context between replays; every
+    replay is checked against std::sort and against the eager result."""
+    max_nseg, max_total = 600, 1 << 21
+    kb = 4
+    d = gqsort.DeviceBuffer(gpu_ctx, max_total * kb)
+    dn = gqsort.DeviceBuffer(gpu_ctx, max_total * kb)
+    od = gqsort.DeviceBuffer(gpu_ctx, (max_nseg + 1) * 8)
+    nd = gqsort.DeviceBuffer(gpu_ctx, 4)
+    stream = gqsort.Stream(gpu_ctx)
+    g = gqsort.SegmentedGraph(gpu_ctx, gqsort.U32, d, dn, od, nd, max_nseg, max_total)
+    try:
+        for rep, nseg in enumerate([600, 250, 600, 40]):
+            lens = gqsort.batch_lengths(nseg, 16, 20000, seed=100 + rep, total=max_total * 0.9)
+            keys, offs = make_batch(gqsort.U32, lens, 200 + rep)
+            assert keys.size <= max_total
+            print(f"case graph replay {rep} nseg={nseg} total={keys.size}", flush=True)
+            d.upload(keys, stream)
+            od.upload(offs, stream)
+            nd.upload(np.array([nseg], np.uint32), stream)
+            g.launch(stream)
+            stream.sync()
+            got = d.download(keys.dtype, keys.size, stream)
+            check_batch(got, keys, offs, f"graph replay {rep}")
+            # eager work of the same context between replays
+            e = gqsort.DeviceBuffer.from_array(gpu_ctx, keys)
+            en = gqsort.DeviceBuffer(gpu_ctx, keys.nbytes)
+            eo = gqsort.DeviceBuffer.from_array(gpu_ctx, offs)
+            gpu_ctx.sort_segmented(gqsort.U32, e, en, eo, nseg, keys.size, stream)
+            stream.sync()
+            assert np.array_equal(e.download(keys.dtype, keys.size, stream), got)
+    finally:
+        g.close()
+        stream.close()
