@@ -54,6 +54,16 @@ class Params(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("slot_keys", ctypes.c_uint64), ("slots", ctypes.c_uint32),
+                ("max_batch_requests", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "batches", "max_batch_requests",
+                                               "keys_sorted", "pending")]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("levels_launched", ctypes.c_uint32), ("levels_used", ctypes.c_uint32),
                 ("oversize_leaves", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
@@ -86,6 +96,17 @@ _SIGS = {
                                                    ctypes.POINTER(_P)]),
     "gqs_graph_launch": (ctypes.c_int, [_P, _P]),
     "gqs_graph_destroy": (ctypes.c_int, [_P]),
+    "gqs_server_default_config": (ctypes.c_int, [ctypes.POINTER(ServerConfig)]),
+    "gqs_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+    "gqs_server_submit": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
+    "gqs_server_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int64]),
+    "gqs_server_poll": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "gqs_server_request_times": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "gqs_server_release": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "gqs_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+    "gqs_server_destroy": (ctypes.c_int, [_P]),
+    "gqs_server_get_config": (ctypes.c_int, [_P, ctypes.POINTER(ServerConfig)]),
     "gqs_malloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "gqs_free": (ctypes.c_int, [_P, _P]),
     "gqs_host_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
@@ -388,3 +409,60 @@ def device_sort(ctx, a, stream=None):
     d.free()
     dn.free()
     return out
+
+
+class Server:
+    """Continuous-batching sort server (gqs_server_*)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=None, slots=None, slot_keys=None, max_batch_requests=None):
+        self._lib = load()
+        cfg = ServerConfig()
+        _check(self._lib.gqs_server_default_config(ctypes.byref(cfg)))
+        for k, v in (("hbm_budget_bytes", hbm_budget_bytes), ("slots", slots), ("slot_keys", slot_keys),
+                     ("max_batch_requests", max_batch_requests)):
+            if v is not None:
+                setattr(cfg, k, v)
+        h = _P()
+        _check(self._lib.gqs_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.config = ServerConfig()
+        _check(self._lib.gqs_server_get_config(self._h, ctypes.byref(self.config)))
+
+    def submit(self, dtype, in_ptr, out_ptr, n):
+        """Submit a request over raw host pointers; returns its ticket."""
+        t = ctypes.c_uint64()
+        _check(self._lib.gqs_server_submit(self._h, dtype, in_ptr, out_ptr, n, ctypes.byref(t)))
+        return t.value
+
+    def submit_array(self, a, out=None):
+        out = np.empty_like(a) if out is None else out
+        return self.submit(DTYPE_CODES[a.dtype], a.ctypes.data if a.size else None,
+                           out.ctypes.data if a.size else None, a.size), out
+
+    def wait(self, ticket, timeout_us=-1):
+        _check(self._lib.gqs_server_wait(self._h, ticket, timeout_us))
+
+    def times(self, ticket):
+        s, a, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.gqs_server_request_times(self._h, ticket, ctypes.byref(s), ctypes.byref(a),
+                                                  ctypes.byref(d)))
+        return s.value, a.value, d.value
+
+    def release(self, ticket):
+        _check(self._lib.gqs_server_release(self._h, ticket))
+
+    def stats(self):
+        st = ServerStats()
+        _check(self._lib.gqs_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self._h:
+            self._lib.gqs_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

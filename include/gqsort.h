@@ -141,6 +141,41 @@ gqs_status gqs_device_info(gqs_ctx* ctx, int* cus, int* lds_per_wg, int* wave_si
  * 7 normal floats with 1% wide outliers and 0.1% duplicates. */
 gqs_status gqs_generate(int dtype, int gen, uint64_t seed, void* out, size_t n);
 
+/* ---- Serving: continuous batching of sort requests ------------------------
+ * A server owns a context, `slots` streams each with device-resident key
+ * buffers of slot_keys 8-byte keys and a sort workspace (all inside the HBM
+ * budget), and a scheduler thread.  Whenever a slot is free the scheduler
+ * packs the oldest pending requests of one dtype (up to the slot's capacity)
+ * into one segmented sort: copy in, sort, copy out.  Requests submitted while
+ * other batches run are admitted as soon as a slot frees up.  in/out are host
+ * buffers of n keys (pinned memory gives overlapped copies; in == out allowed);
+ * they must stay valid until the request is done.  Destroy a server only when
+ * no thread waits on it. */
+typedef struct gqs_server gqs_server;
+typedef struct gqs_server_config {
+    uint64_t hbm_budget_bytes;   /* device memory the server may hold (default 64 GiB) */
+    uint64_t slot_keys;          /* keys (8-byte units) per slot buffer; 0: budget / (24 * slots) */
+    uint32_t slots;              /* concurrent batches, 1..16 (default 3) */
+    uint32_t max_batch_requests; /* requests per batch (default 4096) */
+} gqs_server_config;
+typedef struct gqs_server_stats {
+    uint64_t submitted, completed, failed, batches, max_batch_requests, keys_sorted, pending;
+} gqs_server_stats;
+gqs_status gqs_server_default_config(gqs_server_config* cfg);
+gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_server** out);
+/* GQS_EBUSY: the request can never fit a slot. */
+gqs_status gqs_server_submit(gqs_server* s, int dtype, const void* in, void* out, size_t n, uint64_t* ticket);
+/* timeout_us < 0 waits forever; GQS_EBUSY on timeout, GQS_EDEVICE if the request failed. */
+gqs_status gqs_server_wait(gqs_server* s, uint64_t ticket, int64_t timeout_us);
+gqs_status gqs_server_poll(gqs_server* s, uint64_t ticket, int* done);
+/* Steady-clock microseconds of submission, admission into a batch, completion. */
+gqs_status gqs_server_request_times(gqs_server* s, uint64_t ticket, double* submit_us, double* admit_us,
+                                    double* done_us);
+gqs_status gqs_server_release(gqs_server* s, uint64_t ticket); /* forget a finished request */
+gqs_status gqs_server_get_stats(gqs_server* s, gqs_server_stats* out);
+gqs_status gqs_server_get_config(const gqs_server* s, gqs_server_config* out); /* effective (derived) values */
+gqs_status gqs_server_destroy(gqs_server* s);
+
 /* ---- Device plumbing from the library's own HIP runtime -----------------
  * For callers behind a foreign-function layer.  Streams and events are
  * hipStream_t / hipEvent_t passed as void*. */
