@@ -8,12 +8,15 @@ K step times are summed.
 
 Prints one JSON line with the throughput (Mkeys/s), the fraction of the
 one-read-plus-one-write HBM roofline, a per-kernel roofline for the dominant
-kernel (HIP events around each launch, in a separate profiled pass), and a
-CPU baseline (std::sort on one core over a bounded sample, timed here).
+kernel (HIP events around each launch, in a separate profiled pass), a CPU
+baseline (std::sort on one core over a bounded sample, timed here), and the
+batched (captured segmented sort vs per-array plain calls) and serving
+(continuous batching vs plain per-request calls, p50/p99) measurements.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--gen uniform] [--dtype u32]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -97,6 +100,141 @@ def cpu_baseline(dt, gen, seed, sample_n, threads):
             "sample": f"{sample_n} {gen} keys (seed {seed}), std::sort on one core"}, par
 
 
+def bench_batched(ctx, args, stream, ev0, ev1):
+    """65,536 uint32 arrays, lengths floor(2^U), U ~ U[4, 20] (seeded), rescaled
+    to about 2^28 keys in total (the configuration's stated total); one captured
+    segmented sort per batch vs one plain gqs_sort_device per array."""
+    lib = gqsort.load()
+    nseg = args.batch_arrays
+    lens = gqsort.batch_lengths(nseg, 16, 1 << 20, seed=args.seed, total=args.batch_total)
+    total = int(lens.sum())
+    offs = np.zeros(nseg + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    keys = gqsort.generate(gqsort.U32, "uniform", args.seed + 1, total)
+    pristine = gqsort.DeviceBuffer.from_array(ctx, keys, stream)
+    d = gqsort.DeviceBuffer(ctx, keys.nbytes)
+    dn = gqsort.DeviceBuffer(ctx, keys.nbytes)
+    od = gqsort.DeviceBuffer.from_array(ctx, offs, stream)
+    nd = gqsort.DeviceBuffer.from_array(ctx, np.array([nseg], np.uint32), stream)
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            lib.gqs_memcpy_async(ctx.handle, d.ptr, pristine.ptr, keys.nbytes, stream.handle)
+            ev0.record(stream)
+            fn()
+            ev1.record(stream)
+            ts.append(ev0.elapsed_ms(ev1))
+        return float(np.median(ts))
+
+    graph = gqsort.SegmentedGraph(ctx, gqsort.U32, d, dn, od, nd, nseg, total)
+    graph.launch(stream)
+    stream.sync()
+    t_graph = timed(lambda: graph.launch(stream), args.batch_reps)
+    out = d.download(np.uint32, total, stream)
+    ok = True
+    bounds = offs[1:-1].astype(np.int64)
+    inside = np.ones(total - 1, bool)
+    inside[bounds[(bounds > 0) & (bounds < total)] - 1] = False
+    ok &= bool(np.all(out[1:][inside] >= out[:-1][inside]))
+    ok &= int(keys.sum(dtype=np.uint64)) == int(out.sum(dtype=np.uint64))
+    graph.close()
+    t_eager = timed(lambda: ctx.sort_segmented(gqsort.U32, d, dn, od, nseg, total, stream), args.batch_reps)
+
+    def plain():
+        gqsort._check(lib.gqs_sort_each(ctx.handle, gqsort.U32, d.ptr, dn.ptr, offs.ctypes.data, nseg,
+                                        stream.handle), ctx)
+    t_plain = timed(plain, 1)
+    for b in (pristine, d, dn, od, nd):
+        b.free()
+    return {
+        "workload": f"{nseg} uint32 arrays, lengths floor(2^U) U~U[4,20] rescaled to {total} keys, uniform keys",
+        "arrays": nseg, "keys": total,
+        "graph_ms": round(t_graph, 3), "eager_ms": round(t_eager, 3), "plain_ms": round(t_plain, 3),
+        "sorts_per_s": round(nseg / (t_graph / 1e3), 1), "keys_per_s": round(total / (t_graph / 1e3), 1),
+        "plain_sorts_per_s": round(nseg / (t_plain / 1e3), 1),
+        "speedup_vs_plain": round(t_plain / t_graph, 2), "check_sorted_and_checksum": ok,
+    }
+
+
+def bench_serving(args):
+    """Poisson stream of sort requests, sizes floor(2^U) U~U[10,26], dtypes
+    60% u32 / 30% f32 / 10% f64, through the continuous-batching server (64 GiB
+    budget) and through plain per-request gqs_sort, on the same stream."""
+    lib = gqsort.load()
+    rng = np.random.default_rng(args.seed + 7)
+    nreq = args.serve_requests
+    sizes = np.floor(2.0 ** rng.uniform(10, 26, nreq)).astype(np.int64)
+    kinds = rng.choice([gqsort.U32, gqsort.F32, gqsort.F64], size=nreq, p=[0.6, 0.3, 0.1])
+    kb = np.array([gqsort.KEY_BYTES[int(k)] for k in kinds], np.int64)
+    offs = np.zeros(nreq + 1, np.int64)
+    offs[1:] = np.cumsum(sizes * kb)
+    ctx = gqsort.Context(0)
+    ins = ctypes.c_void_p()
+    outs = ctypes.c_void_p()
+    gqsort._check(lib.gqs_host_alloc(ctx.handle, int(offs[-1]), ctypes.byref(ins)), ctx)
+    gqsort._check(lib.gqs_host_alloc(ctx.handle, int(offs[-1]), ctypes.byref(outs)), ctx)
+    for i in range(nreq):
+        a = gqsort.generate(int(kinds[i]), "uniform" if kinds[i] == gqsort.U32 else "normal", 5000 + i, int(sizes[i]))
+        ctypes.memmove(ins.value + int(offs[i]), a.ctypes.data, a.nbytes)
+
+    def run_plain(arrivals):
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            gqsort._check(lib.gqs_sort(ctx.handle, int(kinds[i]), ins.value + int(offs[i]), outs.value + int(offs[i]),
+                                       int(sizes[i])), ctx)
+            lat.append(time.perf_counter() - t0 - arrivals[i])
+        return time.perf_counter() - t0, np.array(lat)
+
+    def run_server(arrivals):
+        with gqsort.Server(0, hbm_budget_bytes=args.serve_budget) as srv:
+            tickets = []
+            t0 = time.perf_counter()
+            for i in range(nreq):
+                while time.perf_counter() - t0 < arrivals[i]:
+                    pass
+                tickets.append(srv.submit(int(kinds[i]), ins.value + int(offs[i]), outs.value + int(offs[i]),
+                                          int(sizes[i])))
+            for t in tickets:
+                srv.wait(t)
+            wall = time.perf_counter() - t0
+            lat = np.array([(lambda s, a, d: d - s)(*srv.times(t)) / 1e6 for t in tickets])
+            st = srv.stats()
+        return wall, lat, st
+
+    # warm both paths once on a few requests
+    for i in range(min(4, nreq)):
+        gqsort._check(lib.gqs_sort(ctx.handle, int(kinds[i]), ins.value + int(offs[i]), outs.value + int(offs[i]),
+                                   int(sizes[i])), ctx)
+    zero = np.zeros(nreq)
+    wall_p, lat_p = run_plain(zero)           # saturated: everything offered at t=0
+    wall_s, lat_s, st_s = run_server(zero)
+    rate = args.serve_rate if args.serve_rate > 0 else 0.8 * nreq / wall_p  # below plain's capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    arrivals -= arrivals[0]
+    wall_pr, lat_pr = run_plain(arrivals)
+    wall_sr, lat_sr, st_r = run_server(arrivals)
+    lib.gqs_host_free(ctx.handle, ins.value)
+    lib.gqs_host_free(ctx.handle, outs.value)
+    ctx.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)  # noqa: E731
+    return {
+        "workload": f"{nreq} requests, sizes floor(2^U) U~U[10,26], dtypes 60/30/10 u32/f32/f64, pinned host buffers",
+        "keys": int(sizes.sum()), "budget_bytes": args.serve_budget,
+        "saturated": {"server_sorts_per_s": round(nreq / wall_s, 1), "plain_sorts_per_s": round(nreq / wall_p, 1),
+                      "speedup": round(wall_p / wall_s, 2), "server_batches": st_s["batches"],
+                      "server_max_batch": st_s["max_batch_requests"]},
+        "poisson": {"offered_per_s": round(rate, 1),
+                    "server": {"sorts_per_s": round(nreq / wall_sr, 1), "p50_ms": pct(lat_sr, 50),
+                               "p99_ms": pct(lat_sr, 99), "batches": st_r["batches"]},
+                    "plain": {"sorts_per_s": round(nreq / wall_pr, 1), "p50_ms": pct(lat_pr, 50),
+                              "p99_ms": pct(lat_pr, 99)}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +247,14 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-arrays", type=int, default=65536)
+    ap.add_argument("--batch-total", type=int, default=1 << 28)
+    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="offered sorts/s (0: 80%% of plain capacity)")
+    ap.add_argument("--serve-budget", type=int, default=64 << 30)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sort shards by sample-sort exchange, which this build does not "
@@ -174,6 +320,16 @@ def main():
         log(f"cpu baseline on a {args.cpu_sample}-key sample")
         cpu, par = cpu_baseline(dt, args.gen, args.seed, args.cpu_sample, threads)
 
+    batched = serving = None
+    if not args.no_batched:
+        for b in (pristine, d, dn):
+            b.free()
+        log("batched segmented sort")
+        batched = bench_batched(ctx, args, stream, ev0, ev1)
+    if not args.no_serving:
+        log("serving stream")
+        serving = bench_serving(args)
+
     sort_frac = (2 * nbytes / (ms_per_step / 1e3)) / HBM_PEAK
     line = {
         "metric": f"Mkeys/s {args.dtype} 2^{int(np.log2(n))} (+% HBM roofline)" if n & (n - 1) == 0 else
@@ -201,6 +357,8 @@ def main():
         "step_ms": {"min": round(min(times), 4), "median": round(float(np.median(times)), 4),
                     "max": round(max(times), 4)},
         "check": check,
+        "batched": batched,
+        "serving": serving,
         "device": info,
     }
     print(json.dumps(line), flush=True)
