@@ -128,13 +128,23 @@ gqs_status Ctx::ensure_workspace(Workspace& ws, uint64_t n, uint32_t nseg_in, co
     return GQS_OK;
 }
 
-Grid Ctx::grid() const {
+gqs_params Ctx::effective_params(int dtype, uint64_t n) const {
+    if (use_tuned && n > 1) {
+        int lg = 0;
+        while ((2ull << lg) <= n) ++lg;
+        auto it = tuned.find(std::make_pair(dtype, lg));
+        if (it != tuned.end()) return it->second;
+    }
+    return params;
+}
+
+Grid Ctx::grid(const gqs_params& p) const {
     Grid g;
-    g.part = cus * (int)params.part_wgs_per_cu;
+    g.part = cus * (int)p.part_wgs_per_cu;
     g.splitters = std::min(1024, cus * 4);
     g.scan = cus * 2;
     g.plan = cus * 2;
-    g.leaf = cus * (int)params.leaf_wgs_per_cu;
+    g.leaf = cus * (int)p.leaf_wgs_per_cu;
     return g;
 }
 
@@ -168,7 +178,7 @@ uint32_t levels_needed(uint64_t n, const gqs_params& p) {
 // [0, n) of d (offsets == nullptr) or nseg arrays delimited by offsets.
 gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
                     uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing) {
-    gqs_params p = params;
+    gqs_params p = effective_params(dtype, n);
     if (capturing) p.host_driven = 0;
     // a work-group leaf holds 32 (u32/f32) or 16 (f64) keys per thread
     const uint32_t cap = (dtype == DT_F64) ? 4096u : 8192u;
@@ -184,7 +194,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         }
         prof.clear();
     }
-    const Grid g = grid();
+    const Grid g = grid(p);
     uint32_t L;
     if (offsets) {
         L = levels_needed(n, p);
@@ -238,6 +248,8 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         la.lv.ctl = &ws.ctl[0];
         la.leaf_max = p.leaf_max;
         la.is_last = 1;
+        la.sort_threshold = p.sort_threshold;
+        la.bitonic_cutoff = p.bitonic_cutoff;
         prof_begin(6, st);
         if (fail(launch_leaf(dtype, la, g, st), "leaf(list)")) return GQS_EDEVICE;
         prof_end(st);
@@ -264,6 +276,8 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         la.lv = a;
         la.leaf_max = p.leaf_max;
         la.is_last = a.is_last;
+        la.sort_threshold = p.sort_threshold;
+        la.bitonic_cutoff = p.bitonic_cutoff;
         prof_begin(6, st);
         if (fail(launch_leaf(dtype, la, g, st), "leaf")) return GQS_EDEVICE;
         prof_end(st);
@@ -319,6 +333,8 @@ gqs_status gqs_default_params(int dtype, gqs_params* out) {
     p.host_driven = 0;
     p.part_wgs_per_cu = 4;
     p.leaf_wgs_per_cu = 4;
+    p.sort_threshold = 256;
+    p.bitonic_cutoff = 512;
     *out = p;
     return GQS_OK;
 }
@@ -377,9 +393,11 @@ gqs_status gqs_set_params(gqs_ctx* c, const gqs_params* p) {
     const bool pow2 = p->max_fanout >= 2 && p->max_fanout <= 256 && (p->max_fanout & (p->max_fanout - 1)) == 0;
     if ((p->mode != GQS_MODE_MULTIPIVOT && p->mode != GQS_MODE_CLASSIC) || p->leaf_max < 16 ||
         p->leaf_max > 8192 || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
-        p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32)
+        p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32 ||
+        p->sort_threshold < 32 || p->sort_threshold > 4096 || p->bitonic_cutoff < 64 || p->bitonic_cutoff > 512)
         return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");
     c->params = *p;
+    c->use_tuned = false;
     return GQS_OK;
 }
 

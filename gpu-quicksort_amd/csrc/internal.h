@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -75,7 +76,11 @@ struct Ctx {
     gqs_status set_error(gqs_status s, const char* msg);
     bool capturing_now = false;
     gqs_status ensure_workspace(Workspace& w, uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
-    Grid grid() const;
+    Grid grid(const gqs_params& p) const;
+    gqs_params effective_params(int dtype, uint64_t n) const;
+    std::map<std::pair<int, int>, gqs_params> tuned;  // (dtype, floor(log2 n)) -> parameters
+    std::map<std::pair<int, int>, double> tuned_ms;
+    bool use_tuned = false;
     void prof_begin(int kind, hipStream_t st);
     void prof_end(hipStream_t st);
     // Enqueue one sort on `st` using workspace `w`.  nseg_dev (optional) is a

@@ -57,6 +57,8 @@ struct LeafArgs {
     LevelArgs lv;            // bucket mode: the level whose buckets to sort
     uint32_t leaf_max;
     uint32_t is_last;
+    uint32_t sort_threshold;   // sub-bucket size the local partition aims for
+    uint32_t bitonic_cutoff;   // largest sub-bucket one wave sorts (<= 512)
 };
 
 struct Grid {

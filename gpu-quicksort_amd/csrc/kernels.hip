@@ -877,7 +877,6 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
 constexpr int kWaveSortMax = 512;
 constexpr int kLeafSplit = 64;  // largest local fan-out
 constexpr int kLeafBigWG = 512;
-constexpr int kLeafTail = 256;  // sub-bucket size the local fan-out aims for
 
 template <class K>
 __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
@@ -1144,7 +1143,7 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
                         ds.raw = 0;
                     }
                 }
-                if (ds.n >= 1) ds.kind = (ds.n <= (uint32_t)kWaveSortMax) ? 1u : (ds.n <= a.leaf_max ? 2u : 3u);
+                if (ds.n >= 1) ds.kind = (ds.n <= a.bitonic_cutoff) ? 1u : (ds.n <= a.leaf_max ? 2u : 3u);
                 if ((ds.kind == 1) == BIG) ds.kind = 0;  // the other kernel's leaf
             }
             desc[tid] = ds;
@@ -1198,7 +1197,7 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
             }
             // 2. splitters from a stratified sample, sorted by wave 0
             uint32_t k = 2;
-            while (k < kLeafSplit && k * kLeafTail < n) k <<= 1;
+            while (k < kLeafSplit && k * a.sort_threshold < n) k <<= 1;
             const uint32_t S = min((uint32_t)kWaveSortMax, 16u * k);
             if (tid <= (uint32_t)kLeafSplit) hist[tid] = 0;
             if (tid < S) {
@@ -1293,7 +1292,7 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
                 const uint32_t b = lane;
                 const uint32_t sz = (b < nb) ? bst[b + 1] - bst[b] : 0u;
                 const bool valid = b < nb && sz > 0 && !beq[b];
-                const uint32_t cls = !valid ? 4u : (sz <= 64 ? 0u : (sz <= 128 ? 1u : (sz <= (uint32_t)kWaveSortMax ? 2u : 3u)));
+                const uint32_t cls = !valid ? 4u : (sz <= 64 ? 0u : (sz <= 128 ? 1u : (sz <= a.bitonic_cutoff ? 2u : 3u)));
 #pragma unroll
                 for (uint32_t c = 0; c < 4; ++c) {
                     const uint64_t mk = __ballot(cls == c);
@@ -1405,7 +1404,7 @@ template <int DT>
 static hipError_t leaf_t(const LeafArgs& a, const Grid& g, hipStream_t st) {
     using K = typename KeyOf<DT>::K;
     hipLaunchKernelGGL((k_leaf<DT, false>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
-    if (a.leaf_max > (uint32_t)kWaveSortMax || a.is_last) {
+    if (a.leaf_max > a.bitonic_cutoff || a.is_last) {
         const size_t smem = (size_t)std::max<uint32_t>(a.leaf_max, kWaveSortMax) * sizeof(K);
         hipLaunchKernelGGL((k_leaf<DT, true>), dim3(g.leaf), dim3(kLeafBigWG), smem, st, a);
     }

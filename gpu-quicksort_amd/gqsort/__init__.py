@@ -48,7 +48,8 @@ class Params(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("leaf_max", ctypes.c_uint32), ("leaf_target", ctypes.c_uint32),
                 ("oversample", ctypes.c_uint32), ("max_fanout", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
                 ("host_driven", ctypes.c_uint32), ("part_wgs_per_cu", ctypes.c_uint32),
-                ("leaf_wgs_per_cu", ctypes.c_uint32)]
+                ("leaf_wgs_per_cu", ctypes.c_uint32), ("sort_threshold", ctypes.c_uint32),
+                ("bitonic_cutoff", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -91,6 +92,10 @@ _SIGS = {
     "gqs_device_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_size_t]),
     "gqs_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _P, ctypes.c_size_t]),
+    "gqs_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.POINTER(Params)]),
+    "gqs_tune_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "gqs_tune_load": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "gqs_sort_each": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, _P]),
     "gqs_graph_create_segmented": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t,
                                                    ctypes.POINTER(_P)]),
@@ -235,6 +240,19 @@ class Context:
     def sort_segmented(self, dtype, d, dn, offsets, nseg, total, stream=None):
         _check(self._lib.gqs_sort_segmented(self._h, dtype, _ptr(d), _ptr(dn), _ptr(offsets), nseg, total,
                                             _stream(stream)), self)
+
+    def autotune(self, dtype, n, gen="uniform", reps=3):
+        """Tune parameters for (dtype, size class of n); returns the best Params."""
+        p = Params()
+        g = GENERATORS[gen] if isinstance(gen, str) else gen
+        _check(self._lib.gqs_autotune(self._h, dtype, n, g, reps, ctypes.byref(p)), self)
+        return p
+
+    def tune_save(self, path):
+        _check(self._lib.gqs_tune_save(self._h, os.fsencode(path)), self)
+
+    def tune_load(self, path):
+        _check(self._lib.gqs_tune_load(self._h, os.fsencode(path)), self)
 
     def stats(self):
         s = Stats()

@@ -65,6 +65,8 @@ typedef struct gqs_params {
     uint32_t host_driven;  /* 1: read the level queue back after each level and stop early (CL12 loop) */
     uint32_t part_wgs_per_cu; /* persistent count/scatter work-groups per CU */
     uint32_t leaf_wgs_per_cu; /* persistent local-sort work-groups per CU */
+    uint32_t sort_threshold;  /* local sort: sub-bucket size its partition aims for (SORT_THRESHOLD role), 32..4096 */
+    uint32_t bitonic_cutoff;  /* local sort: largest sub-bucket sorted by one wave's bitonic network, 64..512 */
 } gqs_params;
 
 /* Counters of the last sort on a context. */
@@ -116,6 +118,17 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* ctx, int dtype, void* d, void* dn
                                       gqs_graph** out);
 gqs_status gqs_graph_launch(gqs_graph* g, void* stream);
 gqs_status gqs_graph_destroy(gqs_graph* g);
+
+/* Autotuner (the reference hand-picks QUICKSORT_BLOCK_SIZE, SORT_THRESHOLD and
+ * work-group sizes per vendor, CL12/Quicksort.h:77-98).  Times sorts of n
+ * synthetic keys (generator `gen`, see gqs_generate) on the context's device by
+ * coordinate descent over the parameters, keeps the fastest set for
+ * (dtype, floor(log2 n)) in the context's table and returns it.  Sorts use the
+ * table entry of their (dtype, size class) when one exists, until the next
+ * gqs_set_params.  The table is saved/loaded as JSON. */
+gqs_status gqs_autotune(gqs_ctx* ctx, int dtype, size_t n, int gen, uint32_t reps, gqs_params* best);
+gqs_status gqs_tune_save(gqs_ctx* ctx, const char* path);
+gqs_status gqs_tune_load(gqs_ctx* ctx, const char* path);
 
 /* Statistics of the last sort (synchronises the context's last stream). */
 gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
