@@ -183,14 +183,20 @@ def _check(st, ctx=None):
         raise GQSError(st, msg)
 
 
-class Context:
-    """Owns a gqs_ctx: device, workspace, tuned parameters."""
+TUNED_TABLE = os.path.join(_HERE, "tuned_mi355x.json")
 
-    def __init__(self, device=0):
+
+class Context:
+    """Owns a gqs_ctx: device, workspace, tuned parameters.  With tuned=True
+    the shipped autotuner table for MI355X is loaded (gqs_tune_load)."""
+
+    def __init__(self, device=0, tuned=True):
         self._lib = load()
         h = _P()
         _check(self._lib.gqs_create(ctypes.byref(h), device))
         self._h = h
+        if tuned and os.path.exists(TUNED_TABLE):
+            self.tune_load(TUNED_TABLE)
 
     def close(self):
         if getattr(self, "_h", None):

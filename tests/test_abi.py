@@ -119,3 +119,33 @@ def test_generator_rejects_bad_combinations():
     buf = np.empty(16, np.uint32)
     assert lib.gqs_generate(gqsort.U32, 7, 1, buf.ctypes.data, 16) == 1
     assert lib.gqs_generate(gqsort.F32, 0, 1, buf.ctypes.data, 16) == 1
+
+
+def test_cpp_dropin_compiles_and_links(tmp_path):
+    """A C++ caller written like the reference's harness builds against
+    include/gqsort.hpp and links libgqsort.so; without a GPU it gets a status,
+    not an abort."""
+    import subprocess
+
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include "gqsort.hpp"\n#include <cstdio>\n#include <vector>\n'
+        "int main() {\n"
+        "  gqs_ctx* ctx = nullptr;\n"
+        "  gqs_status s = gqs_create(&ctx, 0);\n"
+        "  if (s != GQS_OK) { std::printf(\"status %d\\n\", (int)s); return 0; }\n"
+        "  std::vector<uint32_t> d = {3, 1, 2}, dn(3);\n"
+        "  s = GPUQSort(ctx, d.size(), d.data(), dn.data());\n"
+        "  std::printf(\"status %d %u %u %u\\n\", (int)s, d[0], d[1], d[2]);\n"
+        "  gqs_destroy(ctx);\n  return 0;\n}\n")
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(gqsort.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", libdir, "-lgqsort", f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    if os.path.exists("/dev/kfd"):
+        assert r.stdout.strip() == "status 0 1 2 3"
+    else:
+        assert r.stdout.strip() == "status 3"
