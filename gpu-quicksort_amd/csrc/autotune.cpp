@@ -108,7 +108,7 @@ gqs_status gqs_autotune(gqs_ctx* c, int dtype, size_t n, int gen, uint32_t reps,
         {&gqs_params::leaf_target, {0, 0}},  // filled from leaf_max below
         {&gqs_params::max_fanout, {128, 256}},
         {&gqs_params::oversample, {8, 16, 32}},
-        {&gqs_params::sort_threshold, {128, 256, 512}},
+        {&gqs_params::sort_threshold, {8, 16, 32}},
         {&gqs_params::bitonic_cutoff, {256, 512}},
     };
     for (const Axis& ax : axes) {

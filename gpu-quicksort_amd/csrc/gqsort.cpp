@@ -333,7 +333,7 @@ gqs_status gqs_default_params(int dtype, gqs_params* out) {
     p.host_driven = 0;
     p.part_wgs_per_cu = 4;
     p.leaf_wgs_per_cu = 4;
-    p.sort_threshold = 256;
+    p.sort_threshold = (dtype == DT_F64) ? 8 : 16;
     p.bitonic_cutoff = 512;
     *out = p;
     return GQS_OK;
@@ -394,7 +394,7 @@ gqs_status gqs_set_params(gqs_ctx* c, const gqs_params* p) {
     if ((p->mode != GQS_MODE_MULTIPIVOT && p->mode != GQS_MODE_CLASSIC) || p->leaf_max < 16 ||
         p->leaf_max > 8192 || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
         p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32 ||
-        p->sort_threshold < 32 || p->sort_threshold > 4096 || p->bitonic_cutoff < 64 || p->bitonic_cutoff > 512)
+        p->sort_threshold < 4 || p->sort_threshold > 4096 || p->bitonic_cutoff < 64 || p->bitonic_cutoff > 512)
         return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");
     c->params = *p;
     c->use_tuned = false;

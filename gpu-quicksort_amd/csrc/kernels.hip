@@ -876,12 +876,47 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
 //     the whole work-group.
 constexpr int kWaveSortMax = 512;
 constexpr int kLeafSplit = 64;  // largest local fan-out
-constexpr int kLeafBigWG = 512;
-
 template <class K>
 __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
 template <class K>
 __device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
+
+constexpr int kLeafBigWG = 512;
+constexpr int kLeafLocal = 512;   // local buckets of a work-group leaf (one per thread)
+// largest bucket one thread sorts in registers (64 VGPRs of keys at most)
+template <class K>
+constexpr uint32_t lane_sort_max() { return sizeof(K) == 4 ? 32u : 16u; }
+
+// Batcher's odd-even merge sort network on N (power of two) registers.
+template <class K, int N>
+__device__ __forceinline__ void oem_network(K (&v)[N]) {
+#pragma unroll
+    for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+        for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+            for (int j = k % p; j + k < N; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i < k; ++i)
+                    if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                        const K lo = kmin(v[i + j], v[i + j + k]), hi = kmax(v[i + j], v[i + j + k]);
+                        v[i + j] = lo;
+                        v[i + j + k] = hi;
+                    }
+}
+
+// One thread sorts A[lo, lo + c) (c <= N) in registers.
+template <class K, int N>
+__device__ __forceinline__ void lane_sort(K* A, uint32_t lo, uint32_t c) {
+    K v[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) v[r] = ((uint32_t)r < c) ? A[lo + r] : key_max<K>();
+    oem_network<K, N>(v);
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+        if ((uint32_t)r < c) A[lo + r] = v[r];
+}
+
 
 // Value of `x` held by lane (lane ^ lm).  The masks the bitonic network needs
 // map to DPP row/quad permutes where one exists (quad_perm for 1, 2, 3;
@@ -1112,12 +1147,15 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
     __shared__ K sK[kLeafSplit];
     __shared__ uint16_t sMeta[kLeafSplit];
     __shared__ uint8_t beq[kLeafSplit + 1];  // splitter index + 1 of an equality sub-bucket
-    __shared__ uint32_t hist[kLeafSplit + 1];
-    __shared__ uint32_t bst[kLeafSplit + 1];
+    __shared__ uint32_t hist[kLeafLocal + 1];
+    __shared__ uint32_t bst[kLeafLocal + 1];
     __shared__ uint32_t s_nb, s_depth, s_nbig;
     __shared__ uint8_t bigs[kLeafSplit];
     __shared__ uint8_t tl_list[4][kLeafSplit];  // sub-buckets per size class
     __shared__ uint32_t tl_n[4];
+    __shared__ K wmin[NW], wmax[NW];
+    __shared__ uint32_t scanw[NW];
+    __shared__ uint16_t bigs16[kLeafLocal];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
@@ -1185,175 +1223,116 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
                 __syncthreads();
                 continue;
             }
-            // 1. the leaf's keys into registers (loads in flight during step 2)
+            // 1. the leaf's keys into registers, and their minimum and maximum
             K kv[KPT];
-            uint32_t kbr[KPT];  // bucket | rank << 8
+            uint32_t kbr[KPT];  // local bucket | rank << 9
+            K mn = key_max<K>(), mx = K(0);
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
                 const uint32_t i = j * NT + tid;
                 kbr[j] = 0;
                 kv[j] = key_max<K>();
-                if (i < n) kv[j] = load_key<DT>(src, i, raw);
+                if (i < n) {
+                    kv[j] = load_key<DT>(src, i, raw);
+                    mn = kmin(mn, kv[j]);
+                    mx = kmax(mx, kv[j]);
+                }
             }
-            // 2. splitters from a stratified sample, sorted by wave 0
-            uint32_t k = 2;
-            while (k < kLeafSplit && k * a.sort_threshold < n) k <<= 1;
-            const uint32_t S = min((uint32_t)kWaveSortMax, 16u * k);
-            if (tid <= (uint32_t)kLeafSplit) hist[tid] = 0;
-            if (tid < S) {
-                const uint64_t h = hash32(0x1eafu, (uint32_t)ds.start, tid) % n;
-                A[tid] = load_key<DT>(src, ((uint64_t)tid * n + h) / S, raw);
+#pragma unroll
+            for (int dd = 32; dd >= 1; dd >>= 1) {
+                mn = kmin(mn, shfl_xor_k(mn, dd));
+                mx = kmax(mx, shfl_xor_k(mx, dd));
             }
+            if (lane == 0) {
+                wmin[wave] = mn;
+                wmax[wave] = mx;
+            }
+            if (tid < kLeafLocal) hist[tid] = 0;
+            if (tid == 0) s_nbig = 0;
             __syncthreads();
-            if (wave == 0) {
-                K v[8];
+            mn = wmin[0];
+            mx = wmax[0];
 #pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const uint32_t j = lane * 8 + r;
-                    v[r] = (j < S) ? A[j] : key_max<K>();
-                }
-                wave_bitonic<K, 8>(v);
-                __builtin_amdgcn_wave_barrier();
+            for (int w = 1; w < NW; ++w) {
+                mn = kmin(mn, wmin[w]);
+                mx = kmax(mx, wmax[w]);
+            }
+            if (mn == mx) {  // constant leaf
+                const K v = from_key<DT>(mn);
+                for (uint32_t i = tid; i < n; i += NT) out[i] = v;
+                __syncthreads();
+                continue;
+            }
+            // 2. local buckets by interpolation between min and max (monotone in
+            //    the key), about sort_threshold keys each, one per thread
+            uint32_t nb = 2;
+            while (nb < (uint32_t)kLeafLocal && nb * a.sort_threshold < n) nb <<= 1;
+            const double scale = (double)nb / ((double)(mx - mn) + 1.0);
 #pragma unroll
-                for (int r = 0; r < 8; ++r) A[lane * 8 + r] = v[r];
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                // candidate i (i < k-1) in lane i; unique splitters with equality flags
-                const uint32_t nc = k - 1;
-                K c = key_max<K>();
-                if (lane < nc) c = A[((lane + 1) * S) / k];
-                const K cprev = __shfl_up(c, 1, 64);
-                const K cnext = __shfl_down(c, 1, 64);
-                const bool first = lane < nc && (lane == 0 || cprev != c);
-                const bool eq = first && lane + 1 < nc && cnext == c;
-                const uint64_t fm = __ballot(first), em = __ballot(eq);
-                const uint32_t m = (uint32_t)__popcll(fm), neq = (uint32_t)__popcll(em);
-                const uint32_t uidx = mbcnt(fm), ebefore = mbcnt(em);
-                beq[lane] = 0;
-                if (lane == 0) beq[kLeafSplit] = 0;
-                __builtin_amdgcn_wave_barrier();
-                if (first) {
-                    sK[uidx] = c;
-                    sMeta[uidx] = (uint16_t)((uidx + ebefore) | ((eq ? 1u : 0u) << 8));
-                    if (eq) beq[uidx + ebefore + 1] = (uint8_t)(uidx + 1);
-                }
-                const uint32_t nb = m + 1 + neq;
-                if (lane >= m) {
-                    sK[lane] = key_max<K>();
-                    sMeta[lane] = (uint16_t)((lane == m) ? (m + neq) : (nb - 1));
-                }
-                if (lane == 0) {
-                    uint32_t depth = 0;
-                    while (((1u << depth) - 1u) < m) ++depth;
-                    s_nb = nb;
-                    s_depth = depth;
-                    s_nbig = 0;
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t i = j * NT + tid;
+                if (i < n) {
+                    uint32_t b;
+                    if constexpr (sizeof(K) == 4) b = (uint32_t)((float)(kv[j] - mn) * (float)scale);
+                    else b = (uint32_t)((double)(kv[j] - mn) * scale);
+                    b = min(b, nb - 1);
+                    kbr[j] = b | (hist_rank(hist, b) << 9);
                 }
             }
             __syncthreads();
-            const uint32_t nb = s_nb, depth = s_depth;
-            // classify and rank
-            {
-                uint32_t bj[KPT];
-                classify_n<K, KPT>(kv, bj, sK, sMeta, depth);
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) {
-                    const uint32_t i = j * NT + tid;
-                    if (i < n) kbr[j] = bj[j] | (hist_rank(hist, bj[j]) << 8);
-                }
-            }
-            __syncthreads();
-            // 3. bucket starts (wave 0 scans nb <= 64 counts)
-            if (wave == 0) {
-                const uint32_t hv = (lane < nb) ? hist[lane] : 0u;
-                uint32_t x = hv;
-#pragma unroll
-                for (int dd = 1; dd < 64; dd <<= 1) {
-                    const uint32_t y = __shfl_up(x, dd, 64);
-                    if (lane >= (uint32_t)dd) x += y;
-                }
-                bst[lane] = x - hv;
-                if (lane == 63) bst[64] = x;
-            }
+            // 3. bucket starts: thread t owns local bucket t
+            const uint32_t mycnt = (tid < nb) ? hist[tid] : 0u;
+            uint32_t tot;
+            const uint32_t mystart = block_excl_scan<NT>(mycnt, scanw, &tot);
+            if (tid < nb) bst[tid] = mystart;
             __syncthreads();
             // 4. place keys by bucket
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
                 const uint32_t i = j * NT + tid;
-                if (i < n) A[bst[kbr[j] & 0xFFu] + (kbr[j] >> 8)] = kv[j];
+                if (i < n) A[bst[kbr[j] & 0x1FFu] + (kbr[j] >> 9)] = kv[j];
             }
             __syncthreads();
-            // 5. sub-buckets.  Wave 0 sorts them into size classes (one lane
-            //    per sub-bucket, nb <= 64); equality sub-buckets are filled;
-            //    sub-buckets of <= 64 keys are sorted four per wave at once,
-            //    <= 128 two per wave, <= 512 one per wave.
-            if (wave == 0) {
-                const uint32_t b = lane;
-                const uint32_t sz = (b < nb) ? bst[b + 1] - bst[b] : 0u;
-                const bool valid = b < nb && sz > 0 && !beq[b];
-                const uint32_t cls = !valid ? 4u : (sz <= 64 ? 0u : (sz <= 128 ? 1u : (sz <= a.bitonic_cutoff ? 2u : 3u)));
-#pragma unroll
-                for (uint32_t c = 0; c < 4; ++c) {
-                    const uint64_t mk = __ballot(cls == c);
-                    if (cls == c) tl_list[c][mbcnt(mk)] = (uint8_t)b;
-                    if (lane == 0) tl_n[c] = (uint32_t)__popcll(mk);
-                }
-            }
-            __syncthreads();
-            for (uint32_t b = wave; b < nb; b += NW) {
-                if (!beq[b]) continue;
-                const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
-                const K v = from_key<DT>(sK[beq[b] - 1]);
-                for (uint32_t i = lane; i < sz; i += 64) out[lo + i] = v;
-            }
+            // 5. each thread sorts its bucket in registers with an odd-even
+            //    merge network sized to the largest bucket of its wave
             {
-                const uint32_t nA = tl_n[0];
-                for (uint32_t g = wave * 4; g < nA; g += NW * 4) {
-                    uint32_t lo4[4], sz4[4];
+                const bool fits = mycnt <= lane_sort_max<K>();
+                uint32_t wm = fits ? mycnt : 0u;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint32_t b = (g + q < nA) ? tl_list[0][g + q] : 0u;
-                        lo4[q] = bst[b];
-                        sz4[q] = (g + q < nA) ? bst[b + 1] - bst[b] : 0u;
-                    }
-                    wave_sort_batch<DT, 1, 4>(A, out, lo4, sz4);
+                for (int dd = 32; dd >= 1; dd >>= 1) wm = max(wm, (uint32_t)__shfl_xor(wm, dd, 64));
+                const uint32_t c = fits ? mycnt : 0u;
+                if constexpr (sizeof(K) == 4) {
+                    if (wm > 16) lane_sort<K, 32>(A, mystart, c);
+                    else if (wm > 8) lane_sort<K, 16>(A, mystart, c);
+                    else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
+                    else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
+                } else {
+                    if (wm > 8) lane_sort<K, 16>(A, mystart, c);
+                    else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
+                    else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
                 }
-                const uint32_t nB = tl_n[1];
-                for (uint32_t g = wave * 2; g < nB; g += NW * 2) {
-                    uint32_t lo2[2], sz2[2];
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const uint32_t b = (g + q < nB) ? tl_list[1][g + q] : 0u;
-                        lo2[q] = bst[b];
-                        sz2[q] = (g + q < nB) ? bst[b + 1] - bst[b] : 0u;
-                    }
-                    wave_sort_batch<DT, 2, 2>(A, out, lo2, sz2);
-                }
-                const uint32_t nC = tl_n[2];
-                for (uint32_t g = wave; g < nC; g += NW) {
-                    const uint32_t b = tl_list[2][g];
-                    const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
-                    const K* As = A + lo;
-                    K* o = out + lo;
-                    wave_sort_any<K>(
-                        sz, [&](uint32_t i) { return As[i]; }, [&](uint32_t i, K v) { o[i] = from_key<DT>(v); });
-                }
-                if (tid == 0) {
-                    s_nbig = tl_n[3];
-                    for (uint32_t q = 0; q < tl_n[3]; ++q) bigs[q] = tl_list[3][q];
+                if (!fits) bigs16[atomicAdd(&s_nbig, 1u)] = (uint16_t)tid;
+            }
+            __syncthreads();
+            // 6. buckets above kLaneSortMax (skewed or duplicate-heavy leaves):
+            //    one wave each up to bitonic_cutoff, larger ones by the work-group
+            const uint32_t nbig = s_nbig;
+            for (uint32_t q = wave; q < nbig; q += NW) {
+                const uint32_t b = bigs16[q];
+                const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
+                if (sz <= a.bitonic_cutoff) {
+                    K* As = A + lo;
+                    wave_sort_any<K>(sz, [&](uint32_t i) { return As[i]; }, [&](uint32_t i, K v) { As[i] = v; });
                 }
             }
             __syncthreads();
-            // 6. oversized sub-buckets: whole work-group in LDS
-            const uint32_t nbig = s_nbig;
             for (uint32_t q = 0; q < nbig; ++q) {
-                const uint32_t b = bigs[q];
-                const uint32_t lo = bst[b], sz = bst[b + 1] - lo;
-                lds_bitonic<K>(A + lo, sz, tid, NT);
-                for (uint32_t i = tid; i < sz; i += NT) out[lo + i] = from_key<DT>(A[lo + i]);
-                __syncthreads();
+                const uint32_t b = bigs16[q];
+                const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
+                if (sz > a.bitonic_cutoff) lds_bitonic<K>(A + lo, sz, tid, NT);
             }
+            // 7. coalesced copy out
+            for (uint32_t i = tid; i < n; i += NT) out[i] = from_key<DT>(A[i]);
         }
         __syncthreads();
     }

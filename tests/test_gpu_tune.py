@@ -28,7 +28,7 @@ def test_autotune_roundtrip(gpu_ctx, tmp_path):
     gpu_ctx.set_params(gqsort.default_params())
 
 
-@pytest.mark.parametrize("thr,cut", [(32, 64), (128, 256), (4096, 512)])
+@pytest.mark.parametrize("thr,cut", [(4, 64), (16, 256), (64, 512), (4096, 512)])
 def test_leaf_cutoffs(gpu_ctx, thr, cut):
     a = gqsort.generate(gqsort.U32, "uniform", 9, 300001)
     got = gpu_sort(gpu_ctx, a, sort_threshold=thr, bitonic_cutoff=cut)
