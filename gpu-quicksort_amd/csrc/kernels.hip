@@ -30,10 +30,14 @@
 //                range of d, in caller representation, by many work-groups
 //                after the scatter (the pivot fill, :214-216, which the
 //                reference does serially in the last work-group).
-//   k_leaf       persistent over buckets: local sort of every bucket of at most
-//                leaf_max keys (lqsort_kernel, :267-421): a local k-way
-//                partition into LDS, then wave-register bitonic tails, written
-//                to d in caller representation.
+//   k_leaf_*     persistent over buckets: local sort of every bucket of at most
+//                leaf_max keys (lqsort_kernel, :267-421), written to d in caller
+//                representation.  k_leaf_small: one wave per leaf of at most
+//                bitonic_cutoff keys, bitonic network in registers.
+//                k_leaf_big: one work-group per larger leaf, software-pipelined
+//                (next leaf copied global->LDS asynchronously while the current
+//                one is partitioned by interpolation into LDS and its local
+//                buckets sorted in registers by odd-even merge networks).
 //
 // Keys are sorted as unsigned integers: float/double bits go through an
 // order-preserving bijection on the first load and back on the final store.
@@ -77,8 +81,15 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Work-group barrier that orders LDS accesses only: global stores and
+// asynchronous global->LDS copies issued before it stay in flight, where the
+// fence of __syncthreads() would wait for the copies.
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// Wait for every global load, store and global->LDS copy this wave issued.
+__device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Exclusive work-group scan of one value per thread (NT threads).
-template <int NT>
+template <int NT, bool kLdsOnly = false>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     uint32_t x = v;
@@ -88,7 +99,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
         if (lane >= (uint32_t)dd) x += y;
     }
     if (lane == 63) wsum[w] = x;
-    __syncthreads();
+    if constexpr (kLdsOnly) lds_sync();
+    else __syncthreads();
     uint32_t pre = 0, tot = 0;
 #pragma unroll
     for (int i = 0; i < NT / 64; ++i) {
@@ -96,7 +108,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
         pre += (i < (int)w) ? t : 0u;
         tot += t;
     }
-    __syncthreads();
+    if constexpr (kLdsOnly) lds_sync();
+    else __syncthreads();
     *total = tot;
     return pre + x - v;
 }
@@ -121,6 +134,47 @@ __device__ __forceinline__ void hist_add(uint32_t* hist, uint32_t b) {
     }
 }
 
+// LDS atomics written as inline assembly, for kernels with global->LDS copies
+// in flight.  The compiler makes every LDS atomic it sees wait for all
+// outstanding global->LDS copies (it cannot tell which LDS they write); these
+// never touch such a buffer, and written this way they do not wait.  The
+// returning forms wait for their own results inside the asm statement.
+__device__ __forceinline__ uint32_t lds_addr(const uint32_t* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint32_t*)p;
+}
+__device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
+    asm volatile("ds_add_u32 %0, %1" ::"v"(lds_addr(p)), "v"(v) : "memory");
+}
+__device__ __forceinline__ uint32_t lds_add_rtn(uint32_t* p, uint32_t v) {
+    uint32_t r;
+    asm volatile("ds_add_rtn_u32 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)" : "=v"(r) : "v"(lds_addr(p)), "v"(v) : "memory");
+    return r;
+}
+// Four returning LDS adds in one asm statement (each lane: four addresses,
+// one increment), waited for together.  The outputs are early-clobber: the
+// adds complete while later ones still read their inputs.
+__device__ __forceinline__ void lds_add_rtn4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t v,
+                                             uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& r3) {
+    asm volatile(
+        "ds_add_rtn_u32 %0, %4, %8\n\t"
+        "ds_add_rtn_u32 %1, %5, %8\n\t"
+        "ds_add_rtn_u32 %2, %6, %8\n\t"
+        "ds_add_rtn_u32 %3, %7, %8\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3)
+        : "v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(v)
+        : "memory");
+}
+__device__ __forceinline__ void hist_add_asm(uint32_t* hist, uint32_t b) {
+    const uint64_t act = __ballot(1);
+    const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+    const uint64_t same = __ballot(b == b0);
+    if (same == act) {
+        if (mbcnt(act) == 0) lds_add(&hist[b0], (uint32_t)__popcll(act));
+    } else {
+        lds_add(&hist[b], 1u);
+    }
+}
 // Same, returning each lane's rank among the tile's keys of its bucket.
 __device__ __forceinline__ uint32_t hist_rank(uint32_t* hist, uint32_t b) {
     const uint64_t act = __ballot(1);
@@ -1132,111 +1186,210 @@ struct LeafDesc {
     uint32_t raw;
 };
 
-// BIG = false: leaves of at most kWaveSortMax keys, one per wave, four per
-// work-group of 256 threads.  BIG = true: larger leaves, one per work-group
-// of kLeafBigWG threads.  Both scan the same leaf space and skip the other's.
-template <int DT, bool BIG>
-__global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_leaf(LeafArgs a) {
+// Entry idx of the leaf space as 16 raw bytes: a SegRec of the level-0 list
+// or a BucketRec of the level (both: 64-bit start, 32-bit size, 32-bit
+// flags).  Kept raw so that its load can stay in flight until decoded.
+static_assert(sizeof(SegRec) == 16 && sizeof(BucketRec) == 16, "leaf records are 16 bytes");
+__device__ __forceinline__ uint4 leaf_rec(const LeafArgs& a, bool list_mode, uint32_t idx, uint32_t count) {
+    if (idx >= count) return make_uint4(0u, 0u, 0u, 0u);
+    return list_mode ? reinterpret_cast<const uint4*>(a.list)[idx] : reinterpret_cast<const uint4*>(a.lv.bkt)[idx];
+}
+__device__ __forceinline__ LeafDesc leaf_decode(const LeafArgs& a, bool list_mode, uint4 r) {
+    LeafDesc ds{0, 0, 0, 0, 0};
+    const uint64_t start = (uint64_t)r.x | ((uint64_t)r.y << 32);
+    if (list_mode) {
+        ds.start = start;
+        ds.n = r.z;
+        ds.from_dn = (r.w & SEG_SRC_DN) ? 1u : 0u;
+        ds.raw = (r.w & SEG_RAW) ? 1u : 0u;
+    } else if (!(r.z & kBucketDirect) && (r.z <= a.leaf_max || a.is_last)) {
+        ds.start = start;
+        ds.n = r.z;
+        ds.from_dn = r.w;
+    }
+    if (ds.n >= 1) ds.kind = (ds.n <= a.bitonic_cutoff) ? 1u : (ds.n <= a.leaf_max ? 2u : 3u);
+    return ds;
+}
+__device__ __forceinline__ LeafDesc uniform_desc(const LeafDesc& d) {
+    LeafDesc u;
+    u.start = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(d.start >> 32)) << 32) |
+              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)d.start);
+    u.n = __builtin_amdgcn_readfirstlane(d.n);
+    u.kind = __builtin_amdgcn_readfirstlane(d.kind);
+    u.from_dn = __builtin_amdgcn_readfirstlane(d.from_dn);
+    u.raw = __builtin_amdgcn_readfirstlane(d.raw);
+    return u;
+}
+
+// Leaves of at most bitonic_cutoff keys: one per wave, four per work-group of
+// kLeafWG threads, each sorted in its wave's registers.
+template <int DT>
+__global__ __launch_bounds__(kLeafWG) void k_leaf_small(LeafArgs a) {
     using K = typename KeyOf<DT>::K;
-    constexpr int NT = BIG ? kLeafBigWG : kLeafWG;
-    constexpr int NW = NT / 64;
-    constexpr int KPT = (sizeof(K) == 4) ? 8192 / NT : 4096 / NT;  // keys per thread of a work-group leaf
-    extern __shared__ unsigned char leaf_smem[];
-    K* A = reinterpret_cast<K*>(leaf_smem);
     __shared__ LeafDesc desc[4];
-    __shared__ K sK[kLeafSplit];
-    __shared__ uint16_t sMeta[kLeafSplit];
-    __shared__ uint8_t beq[kLeafSplit + 1];  // splitter index + 1 of an equality sub-bucket
-    __shared__ uint32_t hist[kLeafLocal + 1];
-    __shared__ uint32_t bst[kLeafLocal + 1];
-    __shared__ uint32_t s_nb, s_depth, s_nbig;
-    __shared__ uint8_t bigs[kLeafSplit];
-    __shared__ uint8_t tl_list[4][kLeafSplit];  // sub-buckets per size class
-    __shared__ uint32_t tl_n[4];
-    __shared__ K wmin[NW], wmax[NW];
-    __shared__ uint32_t scanw[NW];
-    __shared__ uint16_t bigs16[kLeafLocal];
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
     K* dout = (K*)a.d;
-    constexpr uint32_t G = BIG ? 1u : 4u;  // leaves looked at per iteration
-    for (uint32_t g = blockIdx.x; g * G < count; g += gridDim.x) {
-        if (tid < G) {
-            LeafDesc ds{0, 0, 0, 0, 0};
-            const uint32_t idx = g * G + tid;
-            if (idx < count) {
-                if (list_mode) {
-                    const SegRec r = a.list[idx];
-                    ds.start = r.start;
-                    ds.n = r.n;
-                    ds.from_dn = (r.flags & SEG_SRC_DN) ? 1u : 0u;
-                    ds.raw = (r.flags & SEG_RAW) ? 1u : 0u;
-                } else {
-                    const BucketRec br = a.lv.bkt[idx];
-                    if (!(br.size & kBucketDirect) && (br.size <= a.leaf_max || a.is_last)) {
-                        ds.start = br.start;
-                        ds.n = br.size;
-                        ds.from_dn = br.in_dn;
-                        ds.raw = 0;
-                    }
-                }
-                if (ds.n >= 1) ds.kind = (ds.n <= a.bitonic_cutoff) ? 1u : (ds.n <= a.leaf_max ? 2u : 3u);
-                if ((ds.kind == 1) == BIG) ds.kind = 0;  // the other kernel's leaf
-            }
+    for (uint32_t g = blockIdx.x; g * 4 < count; g += gridDim.x) {
+        if (tid < 4) {
+            LeafDesc ds = leaf_decode(a, list_mode, leaf_rec(a, list_mode, g * 4 + tid, count));
+            if (ds.kind != 1) ds.kind = 0;  // larger leaves: k_leaf_big
             desc[tid] = ds;
         }
         __syncthreads();
-        // Small leaves: one per wave, entirely in registers.
-        if constexpr (!BIG) {
-            const LeafDesc ds = desc[wave];
-            if (ds.kind == 1) {
-                const K* src = (ds.from_dn ? (const K*)a.dn : (const K*)a.d) + ds.start;
-                K* out = dout + ds.start;
-                const bool raw = ds.raw;
-                wave_sort_any<K>(
-                    ds.n, [&](uint32_t i) { return load_key<DT>(src, i, raw); },
-                    [&](uint32_t i, K v) { out[i] = from_key<DT>(v); });
-            }
-        }
-        // Large leaves: the whole work-group.
-        if constexpr (BIG) {
-            const LeafDesc ds = desc[0];
-            if (ds.kind < 2) {
-                __syncthreads();  // desc[] is rewritten by the next iteration
-                continue;
-            }
+        const LeafDesc ds = desc[wave];
+        if (ds.kind == 1) {
             const K* src = (ds.from_dn ? (const K*)a.dn : (const K*)a.d) + ds.start;
             K* out = dout + ds.start;
             const bool raw = ds.raw;
-            const uint32_t n = ds.n;
-            if (ds.kind == 3) {
-                // Unbounded fallback: bitonic network in place in the source buffer.
-                K* gsrc = (K*)src;
-                if (raw)
-                    for (uint32_t i = tid; i < n; i += NT) gsrc[i] = to_key<DT>(gsrc[i]);
-                __threadfence_block();
-                __syncthreads();
-                global_bitonic<K>(gsrc, n, tid, NT);
-                for (uint32_t i = tid; i < n; i += NT) out[i] = from_key<DT>(gsrc[i]);
-                if (tid == 0) atomicAdd(&a.lv.ctl->oversize, 1u);
-                __syncthreads();
-                continue;
+            wave_sort_any<K>(
+                ds.n, [&](uint32_t i) { return load_key<DT>(src, i, raw); },
+                [&](uint32_t i, K v) { out[i] = from_key<DT>(v); });
+        }
+        __syncthreads();  // desc[] is rewritten by the next iteration
+    }
+}
+
+// Asynchronous copy of n keys from global memory to LDS without passing
+// through registers (global_load_lds): wave w moves dwords [64w, 64w + 64) of
+// every NT-dword stripe.  Complete after vm_wait_all() in every wave and a
+// barrier.
+template <class K, int NT>
+__device__ __forceinline__ void copy_to_lds_async(K* dst, const K* src, uint32_t n, uint32_t tid) {
+    const uint32_t ndw = n * (uint32_t)(sizeof(K) / 4);
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* p = reinterpret_cast<uint32_t*>(dst);
+    const uint32_t lane = tid & 63u;
+    for (uint32_t base = __builtin_amdgcn_readfirstlane(tid >> 6) * 64u; base < ndw; base += NT) {
+        if (base + lane < ndw)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(s + base + lane),
+                                             (__attribute__((address_space(3))) void*)(p + base), 4, 0, 0);
+    }
+}
+
+// Leaves above bitonic_cutoff keys, one per work-group of kLeafBigWG threads,
+// software-pipelined over the work-group's leaves g_i = blockIdx.x + i * G:
+// while leaf g_i is sorted (keys in registers, LDS buffer A), the keys of
+// g_{i+1} stream into LDS buffer P, the record of g_{i+2} is in flight, and
+// the sorted g_{i-1} is being written out from A.  Barriers inside the
+// iteration order LDS only (lds_sync), so none of that traffic is drained
+// before the next iteration starts.
+//
+// Sorting one leaf: one interpolation partition into about n/sort_threshold
+// local buckets between the leaf's minimum and maximum (monotone in the key,
+// one bucket per thread), keys placed by bucket in A, then every thread sorts
+// its bucket in registers with an odd-even merge network; buckets above the
+// register capacity (skewed or duplicate-heavy leaves) go to one wave each,
+// or to the work-group in LDS.
+template <int DT>
+__global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
+    using K = typename KeyOf<DT>::K;
+    constexpr int NT = kLeafBigWG;
+    constexpr int NW = NT / 64;
+    constexpr uint32_t CAP = (sizeof(K) == 4) ? 8192u : 4096u;  // largest leaf
+    constexpr int KPT = (int)CAP / NT;                            // keys per thread
+    // The small tables lead their LDS block so that their addresses stay
+    // below 64 KiB, within reach of the immediate offset of LDS instructions.
+    // P is an LDS variable of its own: the compiler then knows that plain LDS
+    // loads and stores of the other variables do not touch what the
+    // asynchronous copies write, and places no wait for the copies before
+    // them (LDS atomics it waits for regardless; those here are inline asm).
+    struct Lds {
+        LeafDesc sd[3];
+        uint32_t hist[kLeafLocal + 1];
+        uint32_t bst[kLeafLocal + 1];
+        uint32_t s_nbig;
+        uint32_t junk[64];  // targets of the cursor adds of lanes without a key, one per lane
+        uint32_t scanw[NW];
+        K wmin[NW], wmax[NW];
+        uint16_t bigs16[kLeafLocal];
+        alignas(16) K A[CAP];  // the leaf being sorted, written out next iteration
+    };
+    __shared__ Lds S;
+    __shared__ K P[CAP];  // raw keys of the next leaf, arriving asynchronously
+    LeafDesc* sd = S.sd;
+    uint32_t* hist = S.hist;
+    uint32_t* bst = S.bst;
+    uint32_t& s_nbig = S.s_nbig;
+    uint32_t* scanw = S.scanw;
+    K* wmin = S.wmin;
+    K* wmax = S.wmax;
+    uint16_t* bigs16 = S.bigs16;
+    K* A = S.A;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const bool list_mode = a.list != nullptr;
+    const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
+    const uint32_t G = gridDim.x;
+    K* dout = (K*)a.d;
+    auto big_only = [](LeafDesc ds) {
+        if (ds.kind < 2) ds.kind = 0;  // the wave kernel's leaves
+        return ds;
+    };
+    // Thread 0: record of g_{i+1}, in flight.  Loaded from a clamped index
+    // and discarded at decode time when past the end, so that no register
+    // of an outstanding load is overwritten (which would wait for it).
+    uint4 rec = make_uint4(0u, 0u, 0u, 0u);
+    auto fetch = [&](uint32_t idx) {
+        if (count) rec = leaf_rec(a, list_mode, min(idx, count - 1u), count);
+    };
+    auto decode = [&](uint32_t idx) {
+        return big_only(leaf_decode(a, list_mode, idx < count ? rec : make_uint4(0u, 0u, 0u, 0u)));
+    };
+    if (tid == 0) {
+        fetch(blockIdx.x);
+        sd[0] = decode(blockIdx.x);
+        fetch(blockIdx.x + G);
+    }
+    lds_sync();
+    {
+        const LeafDesc c = uniform_desc(sd[0]);
+        if (c.kind == 2) copy_to_lds_async<K, NT>(P, (c.from_dn ? (const K*)a.dn : (const K*)a.d) + c.start, c.n, tid);
+    }
+    LeafDesc pend{0, 0, 0, 0, 0};  // sorted in A, not yet written out
+    uint32_t g = blockIdx.x;
+    for (uint32_t i = 0;; ++i, g += G) {
+        vm_wait_all();  // keys of g_i in P, record of g_{i+1}, stores of g_{i-2}
+        if (tid == 0) sd[(i + 1) % 3] = decode(g + G);
+        lds_sync();
+        // the sorted g_{i-1} leaves A after the keys of g_i have left P: a
+        // read of P waits for every store issued before it
+        auto write_out_pending = [&]() {
+            if (pend.kind) {
+                K* out = dout + pend.start;
+                constexpr int U = 8;
+                for (uint32_t q0 = 0; q0 < pend.n; q0 += U * NT) {
+                    K v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) v[u] = A[min(q0 + u * NT + tid, CAP - 1u)];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (q0 + u * NT + tid < pend.n) out[q0 + u * NT + tid] = from_key<DT>(v[u]);
+                }
+                pend.kind = 0;
             }
-            // 1. the leaf's keys into registers, and their minimum and maximum
-            K kv[KPT];
-            uint32_t kbr[KPT];  // local bucket | rank << 9
+        };
+        if (g >= count) {
+            write_out_pending();
+            break;
+        }
+        const LeafDesc cur = uniform_desc(sd[i % 3]);
+        const LeafDesc nxt = uniform_desc(sd[(i + 1) % 3]);
+        // 1. keys of g_i from P into registers, with their minimum and maximum
+        K kv[KPT];
+        if (cur.kind == 2) {
             K mn = key_max<K>(), mx = K(0);
+            // every slot is read (P holds CAP keys), so the reads issue
+            // together; slots past the leaf are masked afterwards
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) kv[j] = P[j * NT + tid];
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
-                const uint32_t i = j * NT + tid;
-                kbr[j] = 0;
-                kv[j] = key_max<K>();
-                if (i < n) {
-                    kv[j] = load_key<DT>(src, i, raw);
-                    mn = kmin(mn, kv[j]);
-                    mx = kmax(mx, kv[j]);
-                }
+                const bool in = j * NT + tid < cur.n;
+                const K v = cur.raw ? to_key<DT>(kv[j]) : kv[j];
+                kv[j] = in ? v : key_max<K>();
+                mn = kmin(mn, kv[j]);
+                mx = kmax(mx, in ? v : K(0));
             }
 #pragma unroll
             for (int dd = 32; dd >= 1; dd >>= 1) {
@@ -1247,96 +1400,126 @@ __global__ __launch_bounds__(BIG ? kLeafBigWG : kLeafWG, BIG ? 4 : 1) void k_lea
                 wmin[wave] = mn;
                 wmax[wave] = mx;
             }
-            if (tid < kLeafLocal) hist[tid] = 0;
-            if (tid == 0) s_nbig = 0;
+        }
+        write_out_pending();
+        if (tid < kLeafLocal) hist[tid] = 0;
+        if (tid == 0) s_nbig = 0;
+        lds_sync();  // P consumed and A written out by every wave
+        if (nxt.kind == 2)
+            copy_to_lds_async<K, NT>(P, (nxt.from_dn ? (const K*)a.dn : (const K*)a.d) + nxt.start, nxt.n, tid);
+        if (tid == 0) fetch(g + 2 * G);
+        if (cur.kind == 3) {
+            // Unbounded fallback: bitonic network in place in the source buffer.
+            K* gsrc = (K*)((cur.from_dn ? (const K*)a.dn : (const K*)a.d) + cur.start);
+            if (cur.raw)
+                for (uint32_t q = tid; q < cur.n; q += NT) gsrc[q] = to_key<DT>(gsrc[q]);
+            __threadfence_block();
             __syncthreads();
-            mn = wmin[0];
-            mx = wmax[0];
+            global_bitonic<K>(gsrc, cur.n, tid, NT);
+            K* out = dout + cur.start;
+            for (uint32_t q = tid; q < cur.n; q += NT) out[q] = from_key<DT>(gsrc[q]);
+            if (tid == 0) atomicAdd(&a.lv.ctl->oversize, 1u);
+            continue;
+        }
+        if (cur.kind != 2) continue;
+        const uint32_t n = cur.n;
+        K mn = wmin[0], mx = wmax[0];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) {
-                mn = kmin(mn, wmin[w]);
-                mx = kmax(mx, wmax[w]);
-            }
-            if (mn == mx) {  // constant leaf
-                const K v = from_key<DT>(mn);
-                for (uint32_t i = tid; i < n; i += NT) out[i] = v;
-                __syncthreads();
-                continue;
-            }
-            // 2. local buckets by interpolation between min and max (monotone in
-            //    the key), about sort_threshold keys each, one per thread
-            uint32_t nb = 2;
-            while (nb < (uint32_t)kLeafLocal && nb * a.sort_threshold < n) nb <<= 1;
-            const double scale = (double)nb / ((double)(mx - mn) + 1.0);
+        for (int w = 1; w < NW; ++w) {
+            mn = kmin(mn, wmin[w]);
+            mx = kmax(mx, wmax[w]);
+        }
+        if (mn == mx) {  // constant leaf
+            const K v = from_key<DT>(mn);
+            K* out = dout + cur.start;
+            for (uint32_t q = tid; q < n; q += NT) out[q] = v;
+            continue;
+        }
+        // 2. local buckets by interpolation between min and max, about
+        //    sort_threshold keys each; counted first, then every key takes
+        //    its slot from a per-bucket cursor (order inside a bucket is free:
+        //    the bucket is sorted next)
+        uint32_t nb = 2;
+        while (nb < (uint32_t)kLeafLocal && nb * a.sort_threshold < n) nb <<= 1;
+        const double scale = (double)nb / ((double)(mx - mn) + 1.0);
+        auto local_bucket = [&](K v) -> uint32_t {
+            uint32_t b;
+            if constexpr (sizeof(K) == 4) b = (uint32_t)((float)(v - mn) * (float)scale);
+            else b = (uint32_t)((double)(v - mn) * scale);
+            return min(b, nb - 1);
+        };
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const uint32_t i = j * NT + tid;
-                if (i < n) {
-                    uint32_t b;
-                    if constexpr (sizeof(K) == 4) b = (uint32_t)((float)(kv[j] - mn) * (float)scale);
-                    else b = (uint32_t)((double)(kv[j] - mn) * scale);
-                    b = min(b, nb - 1);
-                    kbr[j] = b | (hist_rank(hist, b) << 9);
-                }
-            }
-            __syncthreads();
-            // 3. bucket starts: thread t owns local bucket t
-            const uint32_t mycnt = (tid < nb) ? hist[tid] : 0u;
-            uint32_t tot;
-            const uint32_t mystart = block_excl_scan<NT>(mycnt, scanw, &tot);
-            if (tid < nb) bst[tid] = mystart;
-            __syncthreads();
-            // 4. place keys by bucket
+        for (int j = 0; j < KPT; ++j)
+            if (j * NT + tid < n) hist_add_asm(hist, local_bucket(kv[j]));
+        lds_sync();
+        // 3. bucket starts: thread t owns local bucket t; hist becomes the cursor
+        const uint32_t mycnt = (tid < nb) ? hist[tid] : 0u;
+        uint32_t tot;
+        const uint32_t mystart = block_excl_scan<NT, true>(mycnt, scanw, &tot);
+        if (tid < nb) {
+            bst[tid] = mystart;
+            hist[tid] = mystart;
+        }
+        lds_sync();
+        // 4. place keys by bucket
+        static_assert(KPT % 4 == 0, "cursor adds go in fours");
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const uint32_t i = j * NT + tid;
-                if (i < n) A[bst[kbr[j] & 0x1FFu] + (kbr[j] >> 9)] = kv[j];
-            }
-            __syncthreads();
-            // 5. each thread sorts its bucket in registers with an odd-even
-            //    merge network sized to the largest bucket of its wave
-            {
-                const bool fits = mycnt <= lane_sort_max<K>();
-                uint32_t wm = fits ? mycnt : 0u;
+        for (int j = 0; j < KPT; j += 4) {
+            if ((uint32_t)j * NT >= n) break;  // no lane has a key from here on
+            uint32_t ad[4], pos[4];
 #pragma unroll
-                for (int dd = 32; dd >= 1; dd >>= 1) wm = max(wm, (uint32_t)__shfl_xor(wm, dd, 64));
-                const uint32_t c = fits ? mycnt : 0u;
-                if constexpr (sizeof(K) == 4) {
-                    if (wm > 16) lane_sort<K, 32>(A, mystart, c);
-                    else if (wm > 8) lane_sort<K, 16>(A, mystart, c);
-                    else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
-                    else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
-                } else {
-                    if (wm > 8) lane_sort<K, 16>(A, mystart, c);
-                    else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
-                    else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
-                }
-                if (!fits) bigs16[atomicAdd(&s_nbig, 1u)] = (uint16_t)tid;
+            for (int u = 0; u < 4; ++u)
+                ad[u] = (j + u) * NT + tid < n ? lds_addr(&hist[local_bucket(kv[j + u])]) : lds_addr(&S.junk[lane]);
+            lds_add_rtn4(ad[0], ad[1], ad[2], ad[3], 1u, pos[0], pos[1], pos[2], pos[3]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if ((j + u) * NT + tid < n) A[pos[u]] = kv[j + u];
+        }
+        lds_sync();
+        // 5. each thread sorts its bucket in registers with an odd-even merge
+        //    network sized to the largest bucket of its wave
+        {
+            const bool fits = mycnt <= lane_sort_max<K>();
+            uint32_t wm = fits ? mycnt : 0u;
+#pragma unroll
+            for (int dd = 32; dd >= 1; dd >>= 1) wm = max(wm, (uint32_t)__shfl_xor(wm, dd, 64));
+            const uint32_t c = fits ? mycnt : 0u;
+            if constexpr (sizeof(K) == 4) {
+                if (wm > 16) lane_sort<K, 32>(A, mystart, c);
+                else if (wm > 8) lane_sort<K, 16>(A, mystart, c);
+                else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
+                else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
+            } else {
+                if (wm > 8) lane_sort<K, 16>(A, mystart, c);
+                else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
+                else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
             }
-            __syncthreads();
-            // 6. buckets above kLaneSortMax (skewed or duplicate-heavy leaves):
-            //    one wave each up to bitonic_cutoff, larger ones by the work-group
-            const uint32_t nbig = s_nbig;
+            if (!fits) bigs16[lds_add_rtn(&s_nbig, 1u)] = (uint16_t)tid;
+        }
+        lds_sync();
+        // 6. buckets above the register capacity: one wave each up to
+        //    bitonic_cutoff, larger ones by the work-group
+        const uint32_t nbig = s_nbig;
+        if (nbig) {
             for (uint32_t q = wave; q < nbig; q += NW) {
                 const uint32_t b = bigs16[q];
                 const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
                 if (sz <= a.bitonic_cutoff) {
                     K* As = A + lo;
-                    wave_sort_any<K>(sz, [&](uint32_t i) { return As[i]; }, [&](uint32_t i, K v) { As[i] = v; });
+                    wave_sort_any<K>(sz, [&](uint32_t x) { return As[x]; }, [&](uint32_t x, K v) { As[x] = v; });
                 }
             }
-            __syncthreads();
+            lds_sync();
             for (uint32_t q = 0; q < nbig; ++q) {
                 const uint32_t b = bigs16[q];
                 const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
                 if (sz > a.bitonic_cutoff) lds_bitonic<K>(A + lo, sz, tid, NT);
             }
-            // 7. coalesced copy out
-            for (uint32_t i = tid; i < n; i += NT) out[i] = from_key<DT>(A[i]);
         }
-        __syncthreads();
+        pend = cur;  // written out at the top of the next iteration
     }
 }
+
 
 // ------------------------------------------------------------ launchers
 hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n, uint32_t flags,
@@ -1381,12 +1564,9 @@ hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid
 
 template <int DT>
 static hipError_t leaf_t(const LeafArgs& a, const Grid& g, hipStream_t st) {
-    using K = typename KeyOf<DT>::K;
-    hipLaunchKernelGGL((k_leaf<DT, false>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
-    if (a.leaf_max > a.bitonic_cutoff || a.is_last) {
-        const size_t smem = (size_t)std::max<uint32_t>(a.leaf_max, kWaveSortMax) * sizeof(K);
-        hipLaunchKernelGGL((k_leaf<DT, true>), dim3(g.leaf), dim3(kLeafBigWG), smem, st, a);
-    }
+    hipLaunchKernelGGL((k_leaf_small<DT>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
+    if (a.leaf_max > a.bitonic_cutoff || a.is_last)
+        hipLaunchKernelGGL((k_leaf_big<DT>), dim3(g.leaf), dim3(kLeafBigWG), 0, st, a);
     return hipGetLastError();
 }
 
