@@ -88,16 +88,29 @@ __device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\
 // Wait for every global load, store and global->LDS copy this wave issued.
 __device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Inclusive scan over the 64 lanes of a wave with DPP: row_shr 1, 2, 4, 8
+// scan each row of 16 lanes (lanes shifted in from outside the row read 0),
+// then row_bcast:15 adds row r's last lane to row r + 1 (rows 1, 3) and
+// row_bcast:31 adds lane 31 to rows 2 and 3.  Every active lane takes part.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp_add(uint32_t x) {
+    return x + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x = dpp_add<0x111>(x);        // row_shr:1
+    x = dpp_add<0x112>(x);        // row_shr:2
+    x = dpp_add<0x114>(x);        // row_shr:4
+    x = dpp_add<0x118>(x);        // row_shr:8
+    x = dpp_add<0x142, 0xA>(x);   // row_bcast:15
+    x = dpp_add<0x143, 0xC>(x);   // row_bcast:31
+    return x;
+}
+
 // Exclusive work-group scan of one value per thread (NT threads).
 template <int NT, bool kLdsOnly = false>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        uint32_t y = __shfl_up(x, dd, 64);
-        if (lane >= (uint32_t)dd) x += y;
-    }
+    const uint32_t x = wave_incl_scan(v);
     if (lane == 63) wsum[w] = x;
     if constexpr (kLdsOnly) lds_sync();
     else __syncthreads();
@@ -123,6 +136,37 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
 
 // Add one to hist[b] for every active lane.  A wave whose lanes all carry the
 // same bucket (sorted runs, all-equal input) issues one LDS atomic.
+// hist_rank for N keys per thread at once: every key's atomic is issued
+// before any result is waited for.  Invalid keys (valid[j] false) take no
+// slot and get rank 0.
+template <int N>
+__device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[N], const bool (&valid)[N],
+                                            uint32_t (&r)[N]) {
+    uint32_t lead[N];  // first valid lane when the wave shares one bucket, else 64
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint64_t act = __ballot(valid[j]);
+        r[j] = 0;
+        lead[j] = 64;
+        if (act == 0) continue;
+        const uint32_t first = (uint32_t)__builtin_ctzll(act);
+        const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)b[j], first);
+        const bool uni = __ballot(valid[j] && b[j] == b0) == act;
+        if (uni) {
+            lead[j] = first;
+            if (valid[j] && lane_id() == first) r[j] = atomicAdd(&hist[b0], (uint32_t)__popcll(act));
+        } else if (valid[j]) {
+            r[j] = atomicAdd(&hist[b[j]], 1u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (lead[j] < 64) {
+            const uint64_t act = __ballot(valid[j]);
+            r[j] = (uint32_t)__builtin_amdgcn_readlane((int)r[j], lead[j]) + mbcnt(act);
+        }
+    }
+}
 __device__ __forceinline__ void hist_add(uint32_t* hist, uint32_t b) {
     const uint64_t act = __ballot(1);
     const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
@@ -797,7 +841,7 @@ __global__ __launch_bounds__(256) void k_fill(LevelArgs a) {
 
 // ------------------------------------------------------------ k_scatter
 template <int DT>
-__global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
+__global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
     using TS = TileShape<K>;
     constexpr int KPT = TS::kKeysPerThread;
@@ -867,13 +911,11 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
             __syncthreads();
             uint32_t bk[KPT];
             uint32_t rk[KPT];
+            bool valid[KPT];
             classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const uint32_t idx = sub + j * kPartWG + tid;
-                rk[j] = 0;
-                if (idx < hi) rk[j] = hist_rank(L.hist, bk[j]);
-            }
+            for (int j = 0; j < KPT; ++j) valid[j] = sub + j * kPartWG + tid < hi;
+            hist_rank_n<KPT>(L.hist, bk, valid, rk);
             // prefetch the next sub-tile while this one is staged and stored
             K nxt[KPT];
             const uint32_t nsub = sub + TS::kSubKeys;
@@ -889,23 +931,48 @@ __global__ __launch_bounds__(kPartWG) void k_scatter(LevelArgs a) {
             lstart[tid] = ls;
             delta[tid] = my_cursor - ls;
             __syncthreads();
+            // stage by bucket: the lstart reads of all keys issue together
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const uint32_t idx = sub + j * kPartWG + tid;
-                if (idx < hi) {
-                    const uint32_t pos = lstart[bk[j]] + rk[j];
-                    stage[pos] = keys[j];
-                    sbk[pos] = (uint8_t)bk[j];
+            for (int j0 = 0; j0 < KPT; j0 += KPT / 2) {
+                uint32_t ps[KPT / 2];
+#pragma unroll
+                for (int j = 0; j < KPT / 2; ++j) ps[j] = lstart[bk[j0 + j]];
+#pragma unroll
+                for (int j = 0; j < KPT / 2; ++j) {
+                    if (valid[j0 + j]) {
+                        stage[ps[j] + rk[j0 + j]] = keys[j0 + j];
+                        sbk[ps[j] + rk[j0 + j]] = (uint8_t)bk[j0 + j];
+                    }
                 }
             }
             __syncthreads();
-            if (s_anydirect) {
-                for (uint32_t i = tid; i < cnt; i += kPartWG) {
-                    const uint32_t b = sbk[i];
-                    if (!direct[b]) dst[delta[b] + i] = stage[i];  // equality buckets: k_fill
+            // coalesced stores of the staged runs, W slots per thread at a
+            // time: bucket ids, then offsets and keys, each batch of LDS reads
+            // issued together (slots past cnt are read but not stored)
+            const bool anyd = s_anydirect != 0;
+            constexpr int W = 8;
+#pragma unroll
+            for (int u0 = 0; u0 < KPT; u0 += W) {
+                if ((uint32_t)u0 * kPartWG >= cnt) break;
+                uint32_t sb[W], dl[W];
+                K sv[W];
+#pragma unroll
+                for (int u = 0; u < W; ++u) sb[u] = sbk[(u0 + u) * kPartWG + tid];
+#pragma unroll
+                for (int u = 0; u < W; ++u) {
+                    dl[u] = delta[sb[u]];
+                    sv[u] = stage[(u0 + u) * kPartWG + tid];
                 }
-            } else {
-                for (uint32_t i = tid; i < cnt; i += kPartWG) dst[delta[sbk[i]] + i] = stage[i];
+                if (anyd) {
+#pragma unroll
+                    for (int u = 0; u < W; ++u)
+                        if (direct[sb[u]]) dl[u] = 0xFFFFFFFFu;  // equality buckets: k_fill
+                }
+#pragma unroll
+                for (int u = 0; u < W; ++u) {
+                    const uint32_t i = (u0 + u) * kPartWG + tid;
+                    if (i < cnt && dl[u] != 0xFFFFFFFFu) dst[dl[u] + i] = sv[u];
+                }
             }
             my_cursor += hv;
 #pragma unroll
