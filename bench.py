@@ -235,6 +235,37 @@ def bench_serving(args):
     }
 
 
+def bench_inputs(ctx, stream, ev0, ev1, d, dn, pristine, cases, steps):
+    """Throughput of plain sorts over other generators and key types (same
+    device buffers, each input restored from its pristine copy before every
+    sort): Mkeys/s, fraction of the one-read-plus-one-write HBM roofline, and
+    sortedness of the last output."""
+    lib = gqsort.load()
+    out = []
+    for dname, gen, n in cases:
+        dt = DTYPES[dname]
+        kb = gqsort.KEY_BYTES[dt]
+        log(f"input {dname} {gen} n={n}")
+        host = gqsort.generate(dt, gen, 1, n)
+        pristine.upload(host, stream)
+        ts = []
+        for r in range(steps + 1):
+            lib.gqs_memcpy_async(ctx.handle, d.ptr, pristine.ptr, n * kb, stream.handle)
+            ev0.record(stream)
+            ctx.sort_device(dt, d, dn, n, stream)
+            ev1.record(stream)
+            if r:
+                ts.append(ev0.elapsed_ms(ev1))
+        res = d.download(host.dtype, n, stream)
+        ok = bool(np.all(res[1:] >= res[:-1])) if dname == "u32" else bool(
+            np.all(np.diff(res.astype(np.float64)) >= 0))
+        ms = float(np.median(ts))
+        out.append({"dtype": dname, "generator": gen, "n": n, "ms": round(ms, 4),
+                    "Mkeys_s": round(n / ms / 1e3, 1),
+                    "roofline_frac": round(2 * n * kb / (ms / 1e3) / HBM_PEAK, 4), "sorted": ok})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,6 +280,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-inputs", action="store_true", help="skip the other generators and float keys")
+    ap.add_argument("--input-steps", type=int, default=3)
     ap.add_argument("--batch-arrays", type=int, default=65536)
     ap.add_argument("--batch-total", type=int, default=1 << 28)
     ap.add_argument("--batch-reps", type=int, default=5)
@@ -320,6 +353,17 @@ def main():
         log(f"cpu baseline on a {args.cpu_sample}-key sample")
         cpu, par = cpu_baseline(dt, args.gen, args.seed, args.cpu_sample, threads)
 
+    inputs = None
+    if not args.no_inputs and n == 1 << 28 and args.dtype == "u32":
+        # the other u32 generators at the same size, then f32 2^27 and f64 2^26
+        # (N(0,1) with outliers and duplicates): one buffer size, 1 GiB / 512 MiB
+        cases = [("u32", g, n) for g in ("gaussian", "sorted", "reverse", "equal", "bucket")]
+        cases += [("f32", "normal", n >> 1), ("f64", "normal", n >> 2)]
+        inputs = bench_inputs(ctx, stream, ev0, ev1, d, dn, pristine, cases, args.input_steps)
+        if not all(c["sorted"] for c in inputs):
+            log(f"CHECK FAILED: {inputs}")
+            sys.exit(1)
+
     batched = serving = None
     if not args.no_batched:
         for b in (pristine, d, dn):
@@ -357,6 +401,7 @@ def main():
         "step_ms": {"min": round(min(times), 4), "median": round(float(np.median(times)), 4),
                     "max": round(max(times), 4)},
         "check": check,
+        "inputs": inputs,
         "batched": batched,
         "serving": serving,
         "device": info,

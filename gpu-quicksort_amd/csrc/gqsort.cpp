@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -128,12 +129,22 @@ gqs_status Ctx::ensure_workspace(Workspace& ws, uint64_t n, uint32_t nseg_in, co
     return GQS_OK;
 }
 
+// Tuned parameters of the size class floor(log2 n) of this key type, or of
+// the nearest tuned class of the same key type (the lower one on a tie).
 gqs_params Ctx::effective_params(int dtype, uint64_t n) const {
     if (use_tuned && n > 1) {
         int lg = 0;
         while ((2ull << lg) <= n) ++lg;
-        auto it = tuned.find(std::make_pair(dtype, lg));
-        if (it != tuned.end()) return it->second;
+        auto hi = tuned.lower_bound(std::make_pair(dtype, lg));
+        auto best = tuned.end();
+        if (hi != tuned.end() && hi->first.first == dtype) best = hi;
+        if (hi != tuned.begin()) {
+            auto lo = std::prev(hi);
+            if (lo->first.first == dtype &&
+                (best == tuned.end() || lg - lo->first.second <= best->first.second - lg))
+                best = lo;
+        }
+        if (best != tuned.end()) return best->second;
     }
     return params;
 }

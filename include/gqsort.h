@@ -124,8 +124,9 @@ gqs_status gqs_graph_destroy(gqs_graph* g);
  * synthetic keys (generator `gen`, see gqs_generate) on the context's device by
  * coordinate descent over the parameters, keeps the fastest set for
  * (dtype, floor(log2 n)) in the context's table and returns it.  Sorts use the
- * table entry of their (dtype, size class) when one exists, until the next
- * gqs_set_params.  The table is saved/loaded as JSON. */
+ * table entry of their (dtype, size class), or of the nearest tuned size class
+ * of the same dtype, until the next gqs_set_params.  The table is saved/loaded
+ * as JSON. */
 gqs_status gqs_autotune(gqs_ctx* ctx, int dtype, size_t n, int gen, uint32_t reps, gqs_params* best);
 gqs_status gqs_tune_save(gqs_ctx* ctx, const char* path);
 gqs_status gqs_tune_load(gqs_ctx* ctx, const char* path);
