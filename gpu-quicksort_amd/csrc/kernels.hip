@@ -136,45 +136,85 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wsum) {
 
 // Add one to hist[b] for every active lane.  A wave whose lanes all carry the
 // same bucket (sorted runs, all-equal input) issues one LDS atomic.
+// Waves whose keys fall into at most two buckets (sorted or clustered input:
+// runs of nearby keys) update the histogram with one atomic per bucket; other
+// waves with one atomic per lane.  All mask arithmetic is on wave-uniform
+// ballots.
+struct Peel {
+    uint64_t m0, m1;  // lanes of the first and second leader's bucket
+    uint32_t l0, l1;  // leader lanes
+    bool two;         // the wave's valid lanes are m0 | m1
+};
+__device__ __forceinline__ Peel peel2(uint32_t b, bool valid) {
+    Peel p{0, 0, 0, 0, false};
+    const uint64_t act = __ballot(valid);
+    if (act == 0) return p;
+    p.l0 = (uint32_t)__builtin_ctzll(act);
+    const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)b, p.l0);
+    p.m0 = __ballot(valid && b == b0);
+    const uint64_t rest = act & ~p.m0;
+    if (rest == 0) {
+        p.two = true;
+        return p;
+    }
+    p.l1 = (uint32_t)__builtin_ctzll(rest);
+    const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)b, p.l1);
+    p.m1 = __ballot(valid && b == b1);
+    p.two = (p.m0 | p.m1) == act;
+    return p;
+}
+// Whether a wave's keys look clustered: the first key slot of the sub-tile
+// falls into at most two buckets.  Random keys skip the peeling.
+__device__ __forceinline__ bool clustered(uint32_t b, bool valid) { return peel2(b, valid).two; }
+
+__device__ __forceinline__ void hist_add2(uint32_t* hist, uint32_t b, bool valid, bool try_peel) {
+    if (!try_peel) {
+        if (valid) atomicAdd(&hist[b], 1u);
+        return;
+    }
+    const Peel p = peel2(b, valid);
+    const uint32_t lane = lane_id();
+    if (p.two) {
+        if (lane == p.l0) atomicAdd(&hist[b], (uint32_t)__popcll(p.m0));
+        if (p.m1 && lane == p.l1) atomicAdd(&hist[b], (uint32_t)__popcll(p.m1));
+    } else if (valid) {
+        atomicAdd(&hist[b], 1u);
+    }
+}
+
 // hist_rank for N keys per thread at once: every key's atomic is issued
 // before any result is waited for.  Invalid keys (valid[j] false) take no
 // slot and get rank 0.
 template <int N>
 __device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[N], const bool (&valid)[N],
                                             uint32_t (&r)[N]) {
-    uint32_t lead[N];  // first valid lane when the wave shares one bucket, else 64
+    if (!clustered(b[0], valid[0])) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            r[j] = 0;
+            if (valid[j]) r[j] = atomicAdd(&hist[b[j]], 1u);
+        }
+        return;
+    }
+    // as hist_add2: a wave of at most two buckets takes one slot range per
+    // bucket (rank = range base + lanes below in the same bucket) and waits
+    // for that key's atomics at once; other waves leave their per-lane
+    // atomics in flight and the result is the rank
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const uint64_t act = __ballot(valid[j]);
+        const Peel pl = peel2(b[j], valid[j]);
+        const uint32_t lane = lane_id();
         r[j] = 0;
-        lead[j] = 64;
-        if (act == 0) continue;
-        const uint32_t first = (uint32_t)__builtin_ctzll(act);
-        const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)b[j], first);
-        const bool uni = __ballot(valid[j] && b[j] == b0) == act;
-        if (uni) {
-            lead[j] = first;
-            if (valid[j] && lane_id() == first) r[j] = atomicAdd(&hist[b0], (uint32_t)__popcll(act));
+        if (pl.two) {
+            uint32_t base = 0;
+            if (lane == pl.l0) base = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m0));
+            if (pl.m1 && lane == pl.l1) base = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m1));
+            const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)base, pl.l0);
+            const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)base, pl.l1);
+            r[j] = ((pl.m0 >> lane) & 1ull) ? r0 + mbcnt(pl.m0) : r1 + mbcnt(pl.m1);
         } else if (valid[j]) {
             r[j] = atomicAdd(&hist[b[j]], 1u);
         }
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        if (lead[j] < 64) {
-            const uint64_t act = __ballot(valid[j]);
-            r[j] = (uint32_t)__builtin_amdgcn_readlane((int)r[j], lead[j]) + mbcnt(act);
-        }
-    }
-}
-__device__ __forceinline__ void hist_add(uint32_t* hist, uint32_t b) {
-    const uint64_t act = __ballot(1);
-    const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
-    const uint64_t same = __ballot(b == b0);
-    if (same == act) {
-        if (mbcnt(act) == 0) atomicAdd(&hist[b0], (uint32_t)__popcll(act));
-    } else {
-        atomicAdd(&hist[b], 1u);
     }
 }
 
@@ -220,22 +260,6 @@ __device__ __forceinline__ void hist_add_asm(uint32_t* hist, uint32_t b) {
     }
 }
 // Same, returning each lane's rank among the tile's keys of its bucket.
-__device__ __forceinline__ uint32_t hist_rank(uint32_t* hist, uint32_t b) {
-    const uint64_t act = __ballot(1);
-    const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
-    const uint64_t same = __ballot(b == b0);
-    uint32_t r;
-    if (same == act) {
-        const uint32_t below = mbcnt(act);
-        uint32_t base = 0;
-        if (below == 0) base = atomicAdd(&hist[b0], (uint32_t)__popcll(act));
-        base = __builtin_amdgcn_readfirstlane(base);
-        r = base + below;
-    } else {
-        r = atomicAdd(&hist[b], 1u);
-    }
-    return r;
-}
 
 // Splitter classification: i = #{splitters < key} by a branch-free search over
 // 2^depth - 1 sorted entries (padded with the maximum key), then the bucket
@@ -514,15 +538,13 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             for (int i = 0; i < kPartWG / 64; ++i) w = max(w, wsum[i]);
             uint32_t ld = 0;
             while ((1u << ld) < w) ++ld;
-            uint32_t depth = 0;
-            while (((1u << depth) - 1u) < m) ++depth;
             SegPlan p;
             p.tile_begin = 0;
             p.ntiles = (n + TileShape<K>::kTileKeys - 1) / TileShape<K>::kTileKeys;
             p.mat_base = 0;
             p.nbuckets = nb;
             p.bucket_base = 0;
-            p.depth = depth;
+            p.has_eq = neq ? 1u : 0u;
             p.lut_shift = shift;
             p.lut_depth = ld;
             p.lut_lo = (uint64_t)lo;
@@ -610,7 +632,7 @@ __device__ __forceinline__ void load_splitters(const LevelArgs& a, uint32_t s, T
 
 // Classification through the lookup table: the table gives the first
 // candidate for #{splitters < key}, D more binary-search steps finish it.
-template <class K, int D, int N>
+template <class K, int D, bool EQ, int N>
 __device__ __forceinline__ void classify_lut_d(const K (&key)[N], uint32_t (&b)[N], const TileLds<K>& L, K lo,
                                                uint32_t shift) {
     uint32_t i[N];
@@ -626,26 +648,46 @@ __device__ __forceinline__ void classify_lut_d(const K (&key)[N], uint32_t (&b)[
 #pragma unroll
         for (int j = 0; j < N; ++j) i[j] += (L.sK[i[j] + step - 1] < key[j]) ? step : 0u;
     }
+    if constexpr (EQ) {
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const uint32_t meta = L.sMeta[i[j]];
-        b[j] = (meta & 0xFFu) + (((meta >> 8) & 1u) & (uint32_t)(key[j] == L.sK[i[j]]));
+        for (int j = 0; j < N; ++j) {
+            const uint32_t meta = L.sMeta[i[j]];
+            b[j] = (meta & 0xFFu) + (((meta >> 8) & 1u) & (uint32_t)(key[j] == L.sK[i[j]]));
+        }
+    } else {
+        // no equality bucket: the bucket of a key is its search index
+#pragma unroll
+        for (int j = 0; j < N; ++j) b[j] = i[j];
     }
 }
 template <class K, int N>
 __device__ __forceinline__ void classify_lut(const K (&key)[N], uint32_t (&b)[N], const TileLds<K>& L,
                                              const SegPlan& p) {
     const K lo = (K)p.lut_lo;
-    switch (p.lut_depth) {
-        case 0: classify_lut_d<K, 0, N>(key, b, L, lo, p.lut_shift); break;
-        case 1: classify_lut_d<K, 1, N>(key, b, L, lo, p.lut_shift); break;
-        case 2: classify_lut_d<K, 2, N>(key, b, L, lo, p.lut_shift); break;
-        case 3: classify_lut_d<K, 3, N>(key, b, L, lo, p.lut_shift); break;
-        case 4: classify_lut_d<K, 4, N>(key, b, L, lo, p.lut_shift); break;
-        case 5: classify_lut_d<K, 5, N>(key, b, L, lo, p.lut_shift); break;
-        case 6: classify_lut_d<K, 6, N>(key, b, L, lo, p.lut_shift); break;
-        case 7: classify_lut_d<K, 7, N>(key, b, L, lo, p.lut_shift); break;
-        default: classify_lut_d<K, 8, N>(key, b, L, lo, p.lut_shift); break;
+    if (p.has_eq) {
+        switch (p.lut_depth) {
+            case 0: classify_lut_d<K, 0, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 1: classify_lut_d<K, 1, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 2: classify_lut_d<K, 2, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 3: classify_lut_d<K, 3, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 4: classify_lut_d<K, 4, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 5: classify_lut_d<K, 5, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 6: classify_lut_d<K, 6, true, N>(key, b, L, lo, p.lut_shift); break;
+            case 7: classify_lut_d<K, 7, true, N>(key, b, L, lo, p.lut_shift); break;
+            default: classify_lut_d<K, 8, true, N>(key, b, L, lo, p.lut_shift); break;
+        }
+    } else {
+        switch (p.lut_depth) {
+            case 0: classify_lut_d<K, 0, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 1: classify_lut_d<K, 1, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 2: classify_lut_d<K, 2, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 3: classify_lut_d<K, 3, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 4: classify_lut_d<K, 4, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 5: classify_lut_d<K, 5, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 6: classify_lut_d<K, 6, false, N>(key, b, L, lo, p.lut_shift); break;
+            case 7: classify_lut_d<K, 7, false, N>(key, b, L, lo, p.lut_shift); break;
+            default: classify_lut_d<K, 8, false, N>(key, b, L, lo, p.lut_shift); break;
+        }
     }
 }
 
@@ -710,10 +752,11 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
             }
             uint32_t bk[TS::kKeysPerThread];
             classify_lut<K, TS::kKeysPerThread>(keys, bk, L, p);
+            {
+                const bool try_peel = clustered(bk[0], sub + tid < hi);
 #pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) {
-                const uint32_t idx = sub + j * kPartWG + tid;
-                if (idx < hi) hist_add(L.hist, bk[j]);
+                for (int j = 0; j < TS::kKeysPerThread; ++j)
+                    hist_add2(L.hist, bk[j], sub + j * kPartWG + tid < hi, try_peel);
             }
 #pragma unroll
             for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = nxt[j];

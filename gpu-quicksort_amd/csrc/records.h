@@ -44,7 +44,7 @@ struct SegPlan {
     uint32_t mat_base;     // first entry of this segment in the count matrix
     uint32_t nbuckets;     // buckets of this segment (m + 1 + #equality buckets)
     uint32_t bucket_base;  // first bucket of this segment in the level's bucket space
-    uint32_t depth;        // binary-search depth: 2^depth - 1 >= number of splitters
+    uint32_t has_eq;       // 1 when some bucket is an equality bucket (else bucket = search index)
     uint32_t lut_shift;    // lookup table: cell = (key - lut_lo) >> lut_shift
     uint32_t lut_depth;    // search steps after the lookup (max over cells)
     uint64_t lut_lo;       // lowest splitter
