@@ -107,7 +107,9 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 }
 
 // Exclusive work-group scan of one value per thread (NT threads).
-template <int NT, bool kLdsOnly = false>
+// kTrailing = false leaves out the barrier that protects wsum from the next
+// call; callers that pass another barrier before reusing wsum may use it.
+template <int NT, bool kLdsOnly = false, bool kTrailing = true>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t x = wave_incl_scan(v);
@@ -121,8 +123,10 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, 
         pre += (i < (int)w) ? t : 0u;
         tot += t;
     }
-    if constexpr (kLdsOnly) lds_sync();
-    else __syncthreads();
+    if constexpr (kTrailing) {
+        if constexpr (kLdsOnly) lds_sync();
+        else __syncthreads();
+    }
     *total = tot;
     return pre + x - v;
 }
@@ -197,23 +201,28 @@ __device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[
         return;
     }
     // as hist_add2: a wave of at most two buckets takes one slot range per
-    // bucket (rank = range base + lanes below in the same bucket) and waits
-    // for that key's atomics at once; other waves leave their per-lane
-    // atomics in flight and the result is the rank
+    // bucket (rank = range base + lanes below in the same bucket).  All
+    // atomics are issued first; the second pass recomputes the (wave-uniform)
+    // peel instead of keeping it, and reads the bases from the leader lanes.
+    const uint32_t lane = lane_id();
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const Peel pl = peel2(b[j], valid[j]);
-        const uint32_t lane = lane_id();
         r[j] = 0;
         if (pl.two) {
-            uint32_t base = 0;
-            if (lane == pl.l0) base = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m0));
-            if (pl.m1 && lane == pl.l1) base = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m1));
-            const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)base, pl.l0);
-            const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)base, pl.l1);
-            r[j] = ((pl.m0 >> lane) & 1ull) ? r0 + mbcnt(pl.m0) : r1 + mbcnt(pl.m1);
+            if (lane == pl.l0) r[j] = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m0));
+            if (pl.m1 && lane == pl.l1) r[j] = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m1));
         } else if (valid[j]) {
             r[j] = atomicAdd(&hist[b[j]], 1u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const Peel pl = peel2(b[j], valid[j]);
+        if (pl.two) {
+            const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)r[j], pl.l0);
+            const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)r[j], pl.l1);
+            r[j] = ((pl.m0 >> lane) & 1ull) ? r0 + mbcnt(pl.m0) : r1 + mbcnt(pl.m1);
         }
     }
 }
@@ -883,26 +892,38 @@ __global__ __launch_bounds__(256) void k_fill(LevelArgs a) {
 }
 
 // ------------------------------------------------------------ k_scatter
+// Persistent over tiles; every work-group takes a contiguous run of tiles, so
+// consecutive tiles mostly share a segment (splitters, segment record and
+// plan stay put) and thread 0 advances the segment index instead of searching
+// for it.  Per 4 Ki-key sub-tile: classify, rank inside the sub-tile, scan the
+// bucket counts, stage the keys grouped by bucket in LDS, store the runs.
+// The keys of the next sub-tile -- or of the next tile's first sub-tile when
+// it belongs to the same segment -- are loaded while the current one is
+// staged and stored.  Barriers per sub-tile: after the ranks (counts and
+// equality flags complete), inside the scan, after the scan (run starts), after
+// the staging; the next sub-tile's first barrier keeps its writes to shared
+// state behind every wave's stores of this one.
+constexpr int kScatterWG = 512;  // threads of a scatter work-group
 template <int DT>
-__global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
+__global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
     using TS = TileShape<K>;
     constexpr int KPT = TS::kKeysPerThread;
+    constexpr uint32_t kSub = (uint32_t)kScatterWG * KPT;  // keys per staged sub-tile
+    static_assert(TS::kTileKeys % kSub == 0, "a tile is whole sub-tiles");
     __shared__ TileLds<K> L;
     __shared__ uint32_t lstart[kSplStride];
     __shared__ uint32_t delta[kSplStride];  // output offset - staging offset, per bucket
     __shared__ uint8_t direct[kSplStride];
-    __shared__ K stage[TS::kSubKeys];
-    __shared__ uint8_t sbk[TS::kSubKeys];
-    __shared__ uint32_t wsum[kPartWG / 64];
+    __shared__ K stage[kSub];
+    __shared__ uint8_t sbk[kSub];
+    __shared__ uint32_t wsum[kScatterWG / 64];
     __shared__ uint32_t s_anydirect;
+    __shared__ SegRec s_seg;  // the current segment's record and plan
+    __shared__ SegPlan s_plan;
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = a.ctl->nseg;
-    uint32_t cur = 0xFFFFFFFFu;
-    // Each work-group takes a contiguous run of tiles, so consecutive tiles
-    // mostly share a segment (splitters stay in LDS) and thread 0 advances the
-    // segment index instead of searching for it.
     const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
     uint32_t seg_i = 0, seg_next = 0;
@@ -910,6 +931,10 @@ __global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
         seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
         seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
     }
+    if (tid < (uint32_t)kSplStride) L.hist[tid] = 0;  // afterwards zeroed by its owner right after each use
+    uint32_t cur = 0xFFFFFFFFu;
+    K keys[KPT];
+    bool have_keys = false;  // keys[] already hold this tile's first sub-tile
     for (uint32_t t = t0; t < t1; ++t) {
         if (tid == 0) {
             while (t >= seg_next) {
@@ -923,19 +948,22 @@ __global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
         const uint32_t s = L.seg;
         if (s != cur) {
             load_splitters<K>(a, s, L);
+            if (tid == 0) {
+                s_seg = a.segs[s];
+                s_plan = a.plans[s];
+            }
             cur = s;
+            __syncthreads();
         }
-        const SegRec seg = a.segs[s];
-        const SegPlan p = a.plans[s];
+        const SegRec seg = s_seg;
+        const SegPlan p = s_plan;
         const uint32_t tl = t - p.tile_begin;
-        // Thread b owns bucket b: its running output offset inside the segment.
-        uint32_t my_cursor = 0;
+        // Thread b owns bucket b: its output offset inside the segment, and
+        // whether it is an equality bucket (loads consumed after the ranks).
+        uint32_t my_cursor = 0, bsize = 0;
         if (tid < p.nbuckets) {
-            const uint32_t m0 = a.mat[p.mat_base];
-            my_cursor = a.mat[p.mat_base + tid * p.ntiles + tl] - m0;
-            const bool dir = (a.bkt[p.bucket_base + tid].size & kBucketDirect) != 0;
-            direct[tid] = dir ? 1 : 0;
-            if (dir) s_anydirect = 1;
+            my_cursor = a.mat[p.mat_base + tid * p.ntiles + tl] - a.mat[p.mat_base];
+            bsize = a.bkt[p.bucket_base + tid].size;
         }
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         K* dst = ((seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn) + seg.start;
@@ -943,38 +971,56 @@ __global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
         const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
         const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
         const K* base = src + seg.start;
-        K keys[KPT];
+        if (!have_keys) {
 #pragma unroll
-        for (int j = 0; j < KPT; ++j) {
-            const uint32_t idx = lo + j * kPartWG + tid;
-            keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t idx = lo + j * kScatterWG + tid;
+                keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+            }
         }
-        for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
-            L.hist[tid] = 0;
-            __syncthreads();
+        have_keys = false;
+        bool anyd = false;
+        for (uint32_t sub = lo; sub < hi; sub += kSub) {
+            const bool first = sub == lo;
             uint32_t bk[KPT];
             uint32_t rk[KPT];
             bool valid[KPT];
             classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) valid[j] = sub + j * kPartWG + tid < hi;
+            for (int j = 0; j < KPT; ++j) valid[j] = sub + j * kScatterWG + tid < hi;
             hist_rank_n<KPT>(L.hist, bk, valid, rk);
-            // prefetch the next sub-tile while this one is staged and stored
+            // prefetch: the next sub-tile, or the next tile's first sub-tile
+            // when that tile is this work-group's and in the same segment
+            uint32_t nsub = sub + kSub, nhi = hi;
+            bool next_tile = false;
+            if (nsub >= hi) {
+                next_tile = t + 1 < t1 && tl + 1 < p.ntiles;
+                nsub = (tl + 1) * (uint32_t)TS::kTileKeys;
+                nhi = next_tile ? min(nsub + (uint32_t)TS::kTileKeys, seg.n) : 0u;
+            }
             K nxt[KPT];
-            const uint32_t nsub = sub + TS::kSubKeys;
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
-                const uint32_t idx = nsub + j * kPartWG + tid;
-                nxt[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+                const uint32_t idx = nsub + j * kScatterWG + tid;
+                nxt[j] = (idx < nhi) ? load_key<DT>(base, idx, raw) : K(0);
+            }
+            if (first && tid < p.nbuckets) {
+                const bool dir = (bsize & kBucketDirect) != 0;
+                direct[tid] = dir ? 1 : 0;
+                if (dir) s_anydirect = 1;
             }
             __syncthreads();
+            if (first) anyd = s_anydirect != 0;
             uint32_t cnt;
             const uint32_t hv = (tid < p.nbuckets) ? L.hist[tid] : 0u;
-            const uint32_t ls = block_excl_scan<kPartWG>(hv, wsum, &cnt);
-            lstart[tid] = ls;
-            delta[tid] = my_cursor - ls;
+            if (tid < (uint32_t)kSplStride) L.hist[tid] = 0;
+            const uint32_t ls = block_excl_scan<kScatterWG, false, false>(hv, wsum, &cnt);
+            if (tid < (uint32_t)kSplStride) {
+                lstart[tid] = ls;
+                delta[tid] = my_cursor - ls;
+            }
             __syncthreads();
-            // stage by bucket: the lstart reads of all keys issue together
+            // stage by bucket: the lstart reads of half the keys issue together
 #pragma unroll
             for (int j0 = 0; j0 < KPT; j0 += KPT / 2) {
                 uint32_t ps[KPT / 2];
@@ -992,19 +1038,18 @@ __global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
             // coalesced stores of the staged runs, W slots per thread at a
             // time: bucket ids, then offsets and keys, each batch of LDS reads
             // issued together (slots past cnt are read but not stored)
-            const bool anyd = s_anydirect != 0;
             constexpr int W = 8;
 #pragma unroll
             for (int u0 = 0; u0 < KPT; u0 += W) {
-                if ((uint32_t)u0 * kPartWG >= cnt) break;
+                if ((uint32_t)u0 * kScatterWG >= cnt) break;
                 uint32_t sb[W], dl[W];
                 K sv[W];
 #pragma unroll
-                for (int u = 0; u < W; ++u) sb[u] = sbk[(u0 + u) * kPartWG + tid];
+                for (int u = 0; u < W; ++u) sb[u] = sbk[(u0 + u) * kScatterWG + tid];
 #pragma unroll
                 for (int u = 0; u < W; ++u) {
                     dl[u] = delta[sb[u]];
-                    sv[u] = stage[(u0 + u) * kPartWG + tid];
+                    sv[u] = stage[(u0 + u) * kScatterWG + tid];
                 }
                 if (anyd) {
 #pragma unroll
@@ -1013,14 +1058,14 @@ __global__ __launch_bounds__(kPartWG, 4) void k_scatter(LevelArgs a) {
                 }
 #pragma unroll
                 for (int u = 0; u < W; ++u) {
-                    const uint32_t i = (u0 + u) * kPartWG + tid;
+                    const uint32_t i = (u0 + u) * kScatterWG + tid;
                     if (i < cnt && dl[u] != 0xFFFFFFFFu) dst[dl[u] + i] = sv[u];
                 }
             }
             my_cursor += hv;
 #pragma unroll
             for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
-            __syncthreads();
+            have_keys = next_tile;
         }
     }
 }
@@ -1656,7 +1701,10 @@ static hipError_t level_step_t(int step, const LevelArgs& a, const Grid& g, hipS
         case 3: hipLaunchKernelGGL(k_scan_reduce, dim3(g.scan), dim3(256), 0, st, a); break;
         case 4: hipLaunchKernelGGL(k_scan_apply, dim3(g.scan), dim3(256), 0, st, a); break;
         case 5: hipLaunchKernelGGL(k_plan, dim3(g.plan), dim3(256), 0, st, a); break;
-        case 6: hipLaunchKernelGGL(k_scatter<DT>, dim3(g.part), dim3(kPartWG), 0, st, a); break;
+        case 6:
+            hipLaunchKernelGGL(k_scatter<DT>, dim3(std::max(1, g.part * kPartWG / kScatterWG)), dim3(kScatterWG), 0, st,
+                               a);
+            break;
         case 7: hipLaunchKernelGGL(k_fill<DT>, dim3(g.plan), dim3(256), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
