@@ -903,7 +903,7 @@ __global__ __launch_bounds__(256) void k_fill(LevelArgs a) {
 // equality flags complete), inside the scan, after the scan (run starts), after
 // the staging; the next sub-tile's first barrier keeps its writes to shared
 // state behind every wave's stores of this one.
-constexpr int kScatterWG = 512;  // threads of a scatter work-group
+constexpr int kScatterWG = 1024;  // threads of a scatter work-group: one staged sub-tile is a whole tile
 template <int DT>
 __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
