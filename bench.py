@@ -41,17 +41,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per sort of `kernel` on `workload` from the committed rocprofv3
-    table (profiles/traffic.txt), or None when that workload was not profiled."""
+# kernels behind each kind of the library's per-kind timing
+KIND_KERNELS = {"count": ("k_count",), "scatter": ("k_scatter",), "leaf": ("k_leaf_big", "k_leaf_small"),
+                "fill": ("k_fill",)}
+
+
+def measured_traffic(kind, workload):
+    """HBM bytes per sort of the kernels of `kind` on `workload` from the
+    committed rocprofv3 table (profiles/traffic.txt), or None when that
+    workload was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
+    total, found = 0.0, False
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and not line.startswith("#") and f[0] == f"k_{kernel}" and f[1] == workload:
-            return float(f[2])
-    return None
+        if len(f) == 3 and not line.startswith("#") and f[0] in KIND_KERNELS.get(kind, ()) and f[1] == workload:
+            total += float(f[2])
+            found = True
+    return total if found else None
 
 
 def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb, workload):
