@@ -40,7 +40,27 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libgqsort.so")
     os.replace(OUT + ".tmp", OUT)
+    build_cli(verbose)
     return OUT
+
+
+CLI_SRC = os.path.join(HERE, "cli", "quicksort_main.cpp")
+CLI_OUT = os.path.join(HERE, "gqsort", "quicksort")
+
+
+def build_cli(verbose=False):
+    """The command-line driver (reference argv and output lines): host C++ only,
+    linked against libgqsort.so next to it."""
+    cmd = ["g++", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(HERE, "..", "include"), CLI_SRC,
+           "-L", os.path.dirname(OUT), "-lgqsort", "-Wl,-rpath,$ORIGIN", "-o", CLI_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the quicksort driver")
+    os.replace(CLI_OUT + ".tmp", CLI_OUT)
+    return CLI_OUT
 
 
 if __name__ == "__main__":
