@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
+#include <atomic>
 #include <iterator>
 #include <cstdio>
 #include <cstring>
@@ -383,6 +385,7 @@ gqs_status gqs_destroy(gqs_ctx* c) {
             hipEventDestroy(r.b);
         }
         c->ws.release();
+        c->release_lanes();
         hipFree(c->host_d);
         hipFree(c->host_dn);
     }
@@ -435,6 +438,99 @@ gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const ui
     return c->run(c->ws, dtype, d, dn, total_keys, offsets, (uint32_t)nseg, nullptr, (hipStream_t)stream, false);
 }
 
+namespace gqs {
+
+constexpr size_t kStageChunk = 16ull << 20;  // bytes per pinned staging chunk
+constexpr int kHostLanes = 4;                 // copy lanes (host threads, streams)
+
+void Ctx::release_lanes() {
+    for (HostLane& l : lanes) {
+        if (l.st) hipStreamSynchronize(l.st);
+        for (int b = 0; b < 2; ++b) {
+            if (l.ev[b]) hipEventDestroy(l.ev[b]);
+            if (l.buf[b]) hipHostFree(l.buf[b]);
+        }
+        if (l.st) hipStreamDestroy(l.st);
+    }
+    lanes.clear();
+}
+
+// Copies between pageable host memory and the device in kStageChunk pieces
+// through pinned staging buffers.  Chunks go round-robin to kHostLanes lanes;
+// a lane copies one chunk on the host while the DMA engine moves its other
+// one (host->device: fill, then copy; device->host: the next chunk's DMA is
+// issued before the current chunk is drained).  Returns when every byte has
+// arrived.
+gqs_status Ctx::host_copy(char* dev, char* host, size_t bytes, bool to_device) {
+    if (lanes.empty()) {
+        lanes.resize(kHostLanes);
+        for (HostLane& l : lanes) {
+            hipError_t e = hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking);
+            for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+                e = hipEventCreateWithFlags(&l.ev[b], hipEventDisableTiming);
+                if (e == hipSuccess) e = hipHostMalloc(&l.buf[b], kStageChunk, hipHostMallocDefault);
+            }
+            if (e != hipSuccess) {
+                release_lanes();
+                return set_error(GQS_ENOMEM, "host staging buffers");
+            }
+        }
+    }
+    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
+    const int nl = (int)std::min<size_t>(kHostLanes, nch);
+    std::atomic<int> bad{0};
+    auto lane = [&](int li) {
+        if (hipSetDevice(device) != hipSuccess) {
+            bad = 1;
+            return;
+        }
+        HostLane& L = lanes[li];
+        auto chunk = [&](size_t k, size_t* off, size_t* len) {
+            *off = ((size_t)li + k * (size_t)nl) * kStageChunk;
+            *len = std::min(kStageChunk, bytes - *off);
+        };
+        const size_t cnt = (nch - (size_t)li + nl - 1) / nl;  // chunks of this lane
+        hipError_t e = hipSuccess;
+        size_t off, len;
+        if (to_device) {
+            for (size_t k = 0; k < cnt && e == hipSuccess; ++k) {
+                const int b = (int)(k & 1);
+                chunk(k, &off, &len);
+                if (k >= 2) e = hipEventSynchronize(L.ev[b]);  // the DMA out of this buffer is done
+                if (e != hipSuccess) break;
+                std::memcpy(L.buf[b], host + off, len);
+                e = hipMemcpyAsync(dev + off, L.buf[b], len, hipMemcpyHostToDevice, L.st);
+                if (e == hipSuccess) e = hipEventRecord(L.ev[b], L.st);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(L.st);
+        } else {
+            auto issue = [&](size_t k) {
+                size_t o, n;
+                chunk(k, &o, &n);
+                hipError_t r = hipMemcpyAsync(L.buf[k & 1], dev + o, n, hipMemcpyDeviceToHost, L.st);
+                return r == hipSuccess ? hipEventRecord(L.ev[k & 1], L.st) : r;
+            };
+            if (cnt) e = issue(0);
+            for (size_t k = 0; k < cnt && e == hipSuccess; ++k) {
+                if (k + 1 < cnt) e = issue(k + 1);  // its buffer was drained at step k - 1
+                if (e == hipSuccess) e = hipEventSynchronize(L.ev[k & 1]);
+                if (e != hipSuccess) break;
+                chunk(k, &off, &len);
+                std::memcpy(host + off, L.buf[k & 1], len);
+            }
+        }
+        if (e != hipSuccess) bad = 1;
+    };
+    std::vector<std::thread> pool;
+    for (int li = 1; li < nl; ++li) pool.emplace_back(lane, li);
+    lane(0);
+    for (auto& t : pool) t.join();
+    return bad ? set_error(GQS_EDEVICE, to_device ? "host->device staging copy" : "device->host staging copy")
+               : GQS_OK;
+}
+
+}  // namespace gqs
+
 gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
     if (!c) return GQS_EINVAL;
     if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_sort: bad dtype");
@@ -456,13 +552,14 @@ gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
             }
             c->host_cap = bytes;
         }
-        if (c->fail(hipMemcpy(c->host_d, d, bytes, hipMemcpyHostToDevice), "H2D")) return GQS_EDEVICE;
+        if (gqs_status s = c->host_copy((char*)c->host_d, (char*)d, bytes, true)) return s;
     }
     gqs_status s = gqs_sort_device(c, dtype, c->host_d, c->host_dn, n, nullptr);
     if (s) return s;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (c->fail(hipMemcpy(d, c->host_d, bytes, hipMemcpyDeviceToHost), "D2H")) return GQS_EDEVICE;
-    return GQS_OK;
+    // the copy lanes' streams do not wait for the sort's (null) stream
+    if (c->fail(hipStreamSynchronize(nullptr), "sort")) return GQS_EDEVICE;
+    return c->host_copy((char*)c->host_d, (char*)d, bytes, false);
 }
 
 gqs_status gqs_sort_each(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* host_offsets, size_t nseg,

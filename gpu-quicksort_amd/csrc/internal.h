@@ -71,6 +71,16 @@ struct Ctx {
     void* host_d = nullptr;
     void* host_dn = nullptr;
     size_t host_cap = 0;
+    // Host-pointer sorts: copy lanes, each a host thread with its own stream
+    // and two pinned staging chunks (allocated on the first such sort).
+    struct HostLane {
+        void* buf[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        hipStream_t st = nullptr;
+    };
+    std::vector<HostLane> lanes;
+    gqs_status host_copy(char* dev, char* host, size_t bytes, bool to_device);
+    void release_lanes();
 
     bool fail(hipError_t e, const char* what);
     gqs_status set_error(gqs_status s, const char* msg);

@@ -86,6 +86,22 @@ def test_host_pointer_entry(gpu_ctx):
     check(d, inp, "GPUQSort host")
 
 
+@pytest.mark.parametrize("dt,n", [(gqsort.U32, 4 * (4 << 20) + 12345), (gqsort.F64, 3 * (2 << 20) + 7),
+                                  (gqsort.F32, (4 << 20) - 1)])
+def test_host_pointer_staging_chunks(gpu_ctx, dt, n):
+    """Host-pointer sorts are staged through pinned 16 MiB chunks on four copy
+    lanes: sizes that span several chunks and every lane, and end mid-chunk
+    (and one just under a single chunk), come back bit-exact; the second call
+    reuses the staging buffers."""
+    gen = "uniform" if dt == gqsort.U32 else "normal"
+    for rep in range(2):
+        d = gqsort.generate(dt, gen, 30 + rep, n)
+        inp = d.copy()
+        print(f"case host staging dt={dt} n={n} rep={rep}", flush=True)
+        gqsort.GPUQSort(gpu_ctx, n, d, np.empty_like(d))
+        check(d, inp, f"host staging dt={dt} n={n}")
+
+
 @pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
 def test_large_2p24_properties(gpu_ctx, gen):
 
