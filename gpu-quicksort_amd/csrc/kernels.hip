@@ -375,6 +375,103 @@ __device__ void lds_bitonic(K* buf, uint32_t n, uint32_t tid, uint32_t nthreads)
     }
 }
 
+template <class K, int E>
+__device__ __forceinline__ void wave_bitonic(K (&v)[E]);  // below, with the local sort
+
+// One wave sorts the 64*E keys of blk (ascending) in its registers.
+template <class K, int E>
+__device__ __forceinline__ void wave_sort_block(K* blk) {
+    const uint32_t lane = threadIdx.x & 63u;
+    K v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = blk[lane * E + r];
+    wave_bitonic<K, E>(v);
+#pragma unroll
+    for (int r = 0; r < E; ++r) blk[lane * E + r] = v[r];
+}
+
+// Sorts the splitter sample buf[0, S) (S a power of two) with kPartWG
+// threads.  Each wave first sorts its own quarter in registers; the bitonic
+// merges that follow need a work-group barrier only for the stages whose
+// compare distance reaches another wave's quarter -- the shorter ones run
+// inside each wave's block, ordered by the wave's own LDS waits -- so a 4096-key
+// sample takes 6 barriers where lds_bitonic takes 78.
+template <class K>
+__device__ void sample_sort(K* buf, uint32_t S, uint32_t tid) {
+    constexpr uint32_t NW = kPartWG / 64;
+    if (S < 64u * NW || S > 64u * 16u * NW) {
+        lds_bitonic<K>(buf, S, tid, kPartWG);
+        return;
+    }
+    const uint32_t B = S / NW;  // keys per wave block: 64 .. 1024
+    const uint32_t lane = tid & 63u;
+    K* blk = buf + (tid >> 6) * B;
+    auto cas = [](K* x, uint32_t i, uint32_t j) {
+        const K a = x[i], b = x[j];
+        if (b < a) {
+            x[i] = b;
+            x[j] = a;
+        }
+    };
+    // register sorts of at most 16 (4-byte) or 8 (8-byte) keys per lane, then
+    // merges inside the wave's block up to its full size
+    constexpr uint32_t kEMax = sizeof(K) == 8 ? 8u : 16u;
+    const uint32_t sub = min(B, 64u * kEMax);
+    for (uint32_t o = 0; o < B; o += sub) {
+        switch (sub / 64) {
+            case 1: wave_sort_block<K, 1>(blk + o); break;
+            case 2: wave_sort_block<K, 2>(blk + o); break;
+            case 4: wave_sort_block<K, 4>(blk + o); break;
+            case 8: wave_sort_block<K, 8>(blk + o); break;
+            default:
+                if constexpr (kEMax == 16) wave_sort_block<K, 16>(blk + o);
+                break;
+        }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    for (uint32_t k = 2 * sub; k <= B; k <<= 1) {
+        for (uint32_t q = lane; q < B / 2; q += 64) {
+            const uint32_t hk = k >> 1;
+            const uint32_t i = ((q & ~(hk - 1)) << 1) | (q & (hk - 1));
+            cas(blk, i, i ^ (k - 1));
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        for (uint32_t h = k >> 2; h >= 1; h >>= 1) {
+            for (uint32_t q = lane; q < B / 2; q += 64) {
+                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
+                cas(blk, i, i + h);
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = 2 * B; k <= S; k <<= 1) {
+        // first stage of the merge: mirror compare across the two halves
+        for (uint32_t q = tid; q < S / 2; q += kPartWG) {
+            const uint32_t hk = k >> 1;
+            const uint32_t i = ((q & ~(hk - 1)) << 1) | (q & (hk - 1));
+            cas(buf, i, i ^ (k - 1));
+        }
+        __syncthreads();
+        uint32_t h = k >> 2;
+        for (; h >= B; h >>= 1) {  // distances that cross wave blocks
+            for (uint32_t q = tid; q < S / 2; q += kPartWG) {
+                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
+                cas(buf, i, i + h);
+            }
+            __syncthreads();
+        }
+        for (; h >= 1; h >>= 1) {  // inside each wave's block
+            for (uint32_t q = lane; q < B / 2; q += 64) {
+                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
+                cas(blk, i, i + h);
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        }
+        __syncthreads();
+    }
+}
+
 // Same network over a global range (unbounded leaves only; one work-group).
 template <class K>
 __device__ void global_bitonic(K* buf, uint64_t n, uint32_t tid, uint32_t nthreads) {
@@ -469,13 +566,24 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             while (S < k * a.oversample && S < (uint32_t)kSampleMax) S <<= 1;
             while (S > n) S >>= 1;
             while (k > 2 && k > S / 2) k >>= 1;
-            for (uint32_t j = tid; j < S; j += kPartWG) {
-                const uint64_t r = hash32(a.seed, (uint32_t)seg.start ^ (uint32_t)(seg.start >> 32), j) % n;
-                const uint64_t pos = ((uint64_t)j * n + r) / S;
-                samp[j] = load_key<DT>(src, seg.start + pos, raw);
+            // stratified random sample: all of a thread's loads issue together
+            constexpr uint32_t kPer = kSampleMax / kPartWG;
+            K sv[kPer];
+#pragma unroll
+            for (uint32_t e = 0; e < kPer; ++e) {
+                const uint32_t j = tid + e * kPartWG;
+                sv[e] = K(0);
+                if (j < S) {
+                    const uint64_t r = hash32(a.seed, (uint32_t)seg.start ^ (uint32_t)(seg.start >> 32), j) % n;
+                    const uint64_t pos = ((uint64_t)j * n + r) / S;
+                    sv[e] = load_key<DT>(src, seg.start + pos, raw);
+                }
             }
+#pragma unroll
+            for (uint32_t e = 0; e < kPer; ++e)
+                if (tid + e * kPartWG < S) samp[tid + e * kPartWG] = sv[e];
             __syncthreads();
-            lds_bitonic<K>(samp, S, tid, kPartWG);
+            sample_sort<K>(samp, S, tid);
             ncand = k - 1;
             for (uint32_t i = tid; i < ncand; i += kPartWG) cand[i] = samp[((uint64_t)(i + 1) * S) / k];
         }
@@ -1096,22 +1204,36 @@ constexpr int kLeafLocal = 512;   // local buckets of a work-group leaf (one per
 template <class K>
 constexpr uint32_t lane_sort_max() { return sizeof(K) == 4 ? 32u : 16u; }
 
-// Batcher's odd-even merge sort network on N (power of two) registers.
+// Batcher's odd-even merge sort network on N (power of two) registers.  The
+// merge sizes P and compare distances D are template parameters, so after
+// unrolling every register index is a constant (a loop whose trip count
+// depends on an outer loop's counter stays a loop, and v[] is then indexed
+// at run time through s_set_gpr_idx).
+template <class K, int N, int P, int D>
+__device__ __forceinline__ void oem_step(K (&v)[N]) {
+    if constexpr (D >= 1) {
+#pragma unroll
+        for (int j = D % P; j + D < N; j += 2 * D)
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+                if (i + j + D < N && (i + j) / (2 * P) == (i + j + D) / (2 * P)) {
+                    const K lo = kmin(v[i + j], v[i + j + D]), hi = kmax(v[i + j], v[i + j + D]);
+                    v[i + j] = lo;
+                    v[i + j + D] = hi;
+                }
+        oem_step<K, N, P, D / 2>(v);
+    }
+}
+template <class K, int N, int P>
+__device__ __forceinline__ void oem_merge(K (&v)[N]) {
+    if constexpr (P < N) {
+        oem_step<K, N, P, P>(v);
+        oem_merge<K, N, 2 * P>(v);
+    }
+}
 template <class K, int N>
 __device__ __forceinline__ void oem_network(K (&v)[N]) {
-#pragma unroll
-    for (int p = 1; p < N; p <<= 1)
-#pragma unroll
-        for (int k = p; k >= 1; k >>= 1)
-#pragma unroll
-            for (int j = k % p; j + k < N; j += 2 * k)
-#pragma unroll
-                for (int i = 0; i < k; ++i)
-                    if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
-                        const K lo = kmin(v[i + j], v[i + j + k]), hi = kmax(v[i + j], v[i + j + k]);
-                        v[i + j] = lo;
-                        v[i + j + k] = hi;
-                    }
+    oem_merge<K, N, 1>(v);
 }
 
 // One thread sorts A[lo, lo + c) (c <= N) in registers.
@@ -1167,69 +1289,47 @@ __device__ __forceinline__ K shfl_xor_k(K v, uint32_t mask) {
 template <int E> struct Log2 { static constexpr int v = (E <= 1) ? 0 : 1 + Log2<E / 2>::v; };
 template <> struct Log2<1> { static constexpr int v = 0; };
 
-// Ascending bitonic sort of 64*E keys held lane-major (element i = lane*E + r
-// lives in v[r] of lane i/E).  All comparators ascending (mirror first step).
-template <class K, int E>
-__device__ __forceinline__ void wave_bitonic(K (&v)[E]) {
-    constexpr int LE = Log2<E>::v;
-    const uint32_t lane = lane_id();
+// Bitonic network over NB independent problems of 64*E keys each, held
+// lane-major (element i = lane*E + r of problem q lives in v[q][r] of lane
+// i/E), all comparators ascending (mirror first step of each merge).  One wave
+// sorts the NB problems together, so it has NB*E independent compare-exchange
+// chains.  Merge sizes and compare distances are template parameters: every
+// register index and shuffle pattern is a constant after inlining.
+template <class K, int E, int NB, int J>
+__device__ __forceinline__ void wb_half_cleaners(K (&v)[NB][E], uint32_t lane) {
+    if constexpr (J >= 1) {
+        constexpr int LE = Log2<E>::v;
+        if constexpr (J < E) {
 #pragma unroll
-    for (int m = 1; m <= 6 + LE; ++m) {
-        const int k = 1 << m;
-        if (k <= E) {
-#pragma unroll
-            for (int r = 0; r < E; ++r) {
-                const int p = r ^ (k - 1);
-                if (p > r) {
-                    const K lo = kmin(v[r], v[p]), hi = kmax(v[r], v[p]);
-                    v[r] = lo;
-                    v[p] = hi;
-                }
-            }
-        } else {
-            const uint32_t lm = (uint32_t)(k - 1) >> LE;
-            const bool upper = (lane & ((uint32_t)(k >> 1) >> LE)) != 0;
-            K t[E];
-#pragma unroll
-            for (int r = 0; r < E; ++r) t[r] = shfl_xor_k(v[E - 1 - r], lm);
-#pragma unroll
-            for (int r = 0; r < E; ++r) v[r] = upper ? kmax(v[r], t[r]) : kmin(v[r], t[r]);
-        }
-#pragma unroll
-        for (int j = k >> 2; j >= 1; j >>= 1) {
-            if (j < E) {
+            for (int q = 0; q < NB; ++q)
 #pragma unroll
                 for (int r = 0; r < E; ++r) {
-                    if (!(r & j)) {
-                        const K lo = kmin(v[r], v[r + j]), hi = kmax(v[r], v[r + j]);
-                        v[r] = lo;
-                        v[r + j] = hi;
+                    if (!(r & J)) {
+                        const K lo = kmin(v[q][r], v[q][r + J]), hi = kmax(v[q][r], v[q][r + J]);
+                        v[q][r] = lo;
+                        v[q][r + J] = hi;
                     }
                 }
-            } else {
-                const uint32_t lm = (uint32_t)j >> LE;
-                const bool upper = (lane & lm) != 0;
+        } else {
+            constexpr uint32_t lm = (uint32_t)J >> LE;
+            const bool upper = (lane & lm) != 0;
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
 #pragma unroll
                 for (int r = 0; r < E; ++r) {
-                    const K t = shfl_xor_k(v[r], lm);
-                    v[r] = upper ? kmax(v[r], t) : kmin(v[r], t);
+                    const K t = shfl_xor_k(v[q][r], lm);
+                    v[q][r] = upper ? kmax(v[q][r], t) : kmin(v[q][r], t);
                 }
-            }
         }
+        wb_half_cleaners<K, E, NB, J / 2>(v, lane);
     }
 }
-
-// NB independent problems of 64*E keys each, sorted together by one wave
-// (problem q in v[q][*]); the problems' steps interleave, so the wave has NB*E
-// independent compare-exchange chains.
-template <class K, int E, int NB>
-__device__ __forceinline__ void wave_bitonic_nb(K (&v)[NB][E]) {
+template <class K, int E, int NB, int M>
+__device__ __forceinline__ void wb_merges(K (&v)[NB][E], uint32_t lane) {
     constexpr int LE = Log2<E>::v;
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int m = 1; m <= 6 + LE; ++m) {
-        const int k = 1 << m;
-        if (k <= E) {
+    if constexpr (M <= 6 + LE) {
+        constexpr int k = 1 << M;
+        if constexpr (k <= E) {
 #pragma unroll
             for (int q = 0; q < NB; ++q)
 #pragma unroll
@@ -1242,7 +1342,7 @@ __device__ __forceinline__ void wave_bitonic_nb(K (&v)[NB][E]) {
                     }
                 }
         } else {
-            const uint32_t lm = (uint32_t)(k - 1) >> LE;
+            constexpr uint32_t lm = (uint32_t)(k - 1) >> LE;
             const bool upper = (lane & ((uint32_t)(k >> 1) >> LE)) != 0;
             K t[NB][E];
 #pragma unroll
@@ -1254,32 +1354,17 @@ __device__ __forceinline__ void wave_bitonic_nb(K (&v)[NB][E]) {
 #pragma unroll
                 for (int r = 0; r < E; ++r) v[q][r] = upper ? kmax(v[q][r], t[q][r]) : kmin(v[q][r], t[q][r]);
         }
-#pragma unroll
-        for (int j = k >> 2; j >= 1; j >>= 1) {
-            if (j < E) {
-#pragma unroll
-                for (int q = 0; q < NB; ++q)
-#pragma unroll
-                    for (int r = 0; r < E; ++r) {
-                        if (!(r & j)) {
-                            const K lo = kmin(v[q][r], v[q][r + j]), hi = kmax(v[q][r], v[q][r + j]);
-                            v[q][r] = lo;
-                            v[q][r + j] = hi;
-                        }
-                    }
-            } else {
-                const uint32_t lm = (uint32_t)j >> LE;
-                const bool upper = (lane & lm) != 0;
-#pragma unroll
-                for (int q = 0; q < NB; ++q)
-#pragma unroll
-                    for (int r = 0; r < E; ++r) {
-                        const K t = shfl_xor_k(v[q][r], lm);
-                        v[q][r] = upper ? kmax(v[q][r], t) : kmin(v[q][r], t);
-                    }
-            }
-        }
+        wb_half_cleaners<K, E, NB, (k >> 2)>(v, lane);
+        wb_merges<K, E, NB, M + 1>(v, lane);
     }
+}
+template <class K, int E, int NB>
+__device__ __forceinline__ void wave_bitonic_nb(K (&v)[NB][E]) {
+    wb_merges<K, E, NB, 1>(v, lane_id());
+}
+template <class K, int E>
+__device__ __forceinline__ void wave_bitonic(K (&v)[E]) {
+    wave_bitonic_nb<K, E, 1>(reinterpret_cast<K (&)[1][E]>(v));
 }
 
 // One wave sorts up to NB sub-buckets of at most 64*E keys each from LDS:
