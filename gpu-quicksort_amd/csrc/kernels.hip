@@ -375,100 +375,79 @@ __device__ void lds_bitonic(K* buf, uint32_t n, uint32_t tid, uint32_t nthreads)
     }
 }
 
+template <class K>
+__device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
+template <class K>
+__device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
+
 template <class K, int E>
 __device__ __forceinline__ void wave_bitonic(K (&v)[E]);  // below, with the local sort
+template <class K, int E, int NB, int J>
+__device__ __forceinline__ void wb_half_cleaners(K (&v)[NB][E], uint32_t lane);
 
-// One wave sorts the 64*E keys of blk (ascending) in its registers.
+// Sorts buf[0, NW*64*E) with the kPartWG threads (NW waves).  Wave w holds
+// block w (B = 64*E keys, element i = lane*E + r in v[r]) in its registers
+// for the whole sort: it sorts the block there, and in each bitonic merge of
+// blocks only the compare distances that reach another block (the mirror
+// step, and distances >= B) go through LDS -- one store, a barrier and a load
+// of the partner block each -- while every shorter distance runs on the
+// registers.  A 4096-key sample thus takes three LDS exchanges.
 template <class K, int E>
-__device__ __forceinline__ void wave_sort_block(K* blk) {
-    const uint32_t lane = threadIdx.x & 63u;
+__device__ __forceinline__ void sample_sort_blocks(K* buf, uint32_t tid) {
+    constexpr uint32_t NW = kPartWG / 64;
+    constexpr uint32_t B = 64u * E;
+    const uint32_t lane = tid & 63u, w = tid >> 6;
     K v[E];
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = blk[lane * E + r];
+    for (int r = 0; r < E; ++r) v[r] = buf[w * B + lane * E + r];
     wave_bitonic<K, E>(v);
+    auto exchange = [&]() {
+        __syncthreads();  // every wave has finished reading buf
 #pragma unroll
-    for (int r = 0; r < E; ++r) blk[lane * E + r] = v[r];
+        for (int r = 0; r < E; ++r) buf[w * B + lane * E + r] = v[r];
+        __syncthreads();
+    };
+#pragma unroll
+    for (uint32_t k = 2 * B; k <= NW * B; k <<= 1) {
+        // mirror step: element g pairs with g ^ (k-1), i.e. local index
+        // B-1-i of block w ^ (k/B - 1), held by lane 63-lane at E-1-r
+        exchange();
+        {
+            const K* pb = buf + (w ^ (k / B - 1u)) * B + (63u - lane) * E;
+            const bool lower = (w & ((k / B) >> 1)) == 0;
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const K t = pb[E - 1 - r];
+                v[r] = lower ? kmin(v[r], t) : kmax(v[r], t);
+            }
+        }
+#pragma unroll
+        for (uint32_t h = k >> 2; h >= B; h >>= 1) {
+            exchange();
+            const K* pb = buf + (w ^ (h / B)) * B + lane * E;
+            const bool lower = (w & (h / B)) == 0;
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const K t = pb[r];
+                v[r] = lower ? kmin(v[r], t) : kmax(v[r], t);
+            }
+        }
+        wb_half_cleaners<K, E, 1, (int)(B / 2)>(reinterpret_cast<K (&)[1][E]>(v), lane);
+    }
+    exchange();
 }
 
-// Sorts the splitter sample buf[0, S) (S a power of two) with kPartWG
-// threads.  Each wave first sorts its own quarter in registers; the bitonic
-// merges that follow need a work-group barrier only for the stages whose
-// compare distance reaches another wave's quarter -- the shorter ones run
-// inside each wave's block, ordered by the wave's own LDS waits -- so a 4096-key
-// sample takes 6 barriers where lds_bitonic takes 78.
+// Sorts the splitter sample buf[0, S) (S a power of two) with kPartWG threads;
+// buf is sorted and visible to the whole work-group on return.
 template <class K>
 __device__ void sample_sort(K* buf, uint32_t S, uint32_t tid) {
-    constexpr uint32_t NW = kPartWG / 64;
-    if (S < 64u * NW || S > 64u * 16u * NW) {
-        lds_bitonic<K>(buf, S, tid, kPartWG);
-        return;
-    }
-    const uint32_t B = S / NW;  // keys per wave block: 64 .. 1024
-    const uint32_t lane = tid & 63u;
-    K* blk = buf + (tid >> 6) * B;
-    auto cas = [](K* x, uint32_t i, uint32_t j) {
-        const K a = x[i], b = x[j];
-        if (b < a) {
-            x[i] = b;
-            x[j] = a;
-        }
-    };
-    // register sorts of at most 16 (4-byte) or 8 (8-byte) keys per lane, then
-    // merges inside the wave's block up to its full size
-    constexpr uint32_t kEMax = sizeof(K) == 8 ? 8u : 16u;
-    const uint32_t sub = min(B, 64u * kEMax);
-    for (uint32_t o = 0; o < B; o += sub) {
-        switch (sub / 64) {
-            case 1: wave_sort_block<K, 1>(blk + o); break;
-            case 2: wave_sort_block<K, 2>(blk + o); break;
-            case 4: wave_sort_block<K, 4>(blk + o); break;
-            case 8: wave_sort_block<K, 8>(blk + o); break;
-            default:
-                if constexpr (kEMax == 16) wave_sort_block<K, 16>(blk + o);
-                break;
-        }
-    }
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    for (uint32_t k = 2 * sub; k <= B; k <<= 1) {
-        for (uint32_t q = lane; q < B / 2; q += 64) {
-            const uint32_t hk = k >> 1;
-            const uint32_t i = ((q & ~(hk - 1)) << 1) | (q & (hk - 1));
-            cas(blk, i, i ^ (k - 1));
-        }
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        for (uint32_t h = k >> 2; h >= 1; h >>= 1) {
-            for (uint32_t q = lane; q < B / 2; q += 64) {
-                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
-                cas(blk, i, i + h);
-            }
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        }
-    }
-    __syncthreads();
-    for (uint32_t k = 2 * B; k <= S; k <<= 1) {
-        // first stage of the merge: mirror compare across the two halves
-        for (uint32_t q = tid; q < S / 2; q += kPartWG) {
-            const uint32_t hk = k >> 1;
-            const uint32_t i = ((q & ~(hk - 1)) << 1) | (q & (hk - 1));
-            cas(buf, i, i ^ (k - 1));
-        }
-        __syncthreads();
-        uint32_t h = k >> 2;
-        for (; h >= B; h >>= 1) {  // distances that cross wave blocks
-            for (uint32_t q = tid; q < S / 2; q += kPartWG) {
-                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
-                cas(buf, i, i + h);
-            }
-            __syncthreads();
-        }
-        for (; h >= 1; h >>= 1) {  // inside each wave's block
-            for (uint32_t q = lane; q < B / 2; q += 64) {
-                const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
-                cas(blk, i, i + h);
-            }
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        }
-        __syncthreads();
+    switch (S / kPartWG) {
+        case 1: sample_sort_blocks<K, 1>(buf, tid); break;
+        case 2: sample_sort_blocks<K, 2>(buf, tid); break;
+        case 4: sample_sort_blocks<K, 4>(buf, tid); break;
+        case 8: sample_sort_blocks<K, 8>(buf, tid); break;
+        case 16: sample_sort_blocks<K, 16>(buf, tid); break;
+        default: lds_bitonic<K>(buf, S, tid, kPartWG); break;
     }
 }
 
@@ -1193,10 +1172,6 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
 //     the whole work-group.
 constexpr int kWaveSortMax = 512;
 constexpr int kLeafSplit = 64;  // largest local fan-out
-template <class K>
-__device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
-template <class K>
-__device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
 
 constexpr int kLeafBigWG = 512;
 constexpr int kLeafLocal = 512;   // local buckets of a work-group leaf (one per thread)
@@ -1283,6 +1258,17 @@ __device__ __forceinline__ K shfl_xor_k(K v, uint32_t mask) {
         return ((K)hi << 32) | lo;
     } else {
         return xlane32(v, mask);
+    }
+}
+
+// Wave-wide min and max: xor distances D, D/2, ..., 1 as template
+// parameters, so each shuffle's pattern is a constant.
+template <class K, int D>
+__device__ __forceinline__ void wave_minmax(K& mn, K& mx) {
+    if constexpr (D >= 1) {
+        mn = kmin(mn, shfl_xor_k(mn, D));
+        mx = kmax(mx, shfl_xor_k(mx, D));
+        wave_minmax<K, D / 2>(mn, mx);
     }
 }
 
@@ -1631,11 +1617,7 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
                 mn = kmin(mn, kv[j]);
                 mx = kmax(mx, in ? v : K(0));
             }
-#pragma unroll
-            for (int dd = 32; dd >= 1; dd >>= 1) {
-                mn = kmin(mn, shfl_xor_k(mn, dd));
-                mx = kmax(mx, shfl_xor_k(mx, dd));
-            }
+            wave_minmax<K, 32>(mn, mx);
             if (lane == 0) {
                 wmin[wave] = mn;
                 wmax[wave] = mx;
