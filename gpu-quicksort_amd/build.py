@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "gqsort", "libgqsort.so")
 ARCH = os.environ.get("GQS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "gqsort.cpp", "devutil.cpp", "generate.cpp", "serve.cpp", "autotune.cpp"]
+SOURCES = ["kernels.hip", "merge.hip", "gqsort.cpp", "devutil.cpp", "generate.cpp", "serve.cpp", "autotune.cpp"]
 
 # Packed FP32 instructions are disabled for device code; the host pass of
 # hipcc warns that it does not know the feature, which is harmless.

@@ -388,6 +388,7 @@ gqs_status gqs_destroy(gqs_ctx* c) {
         c->release_lanes();
         hipFree(c->host_d);
         hipFree(c->host_dn);
+        hipFree(c->merge_split);
     }
     delete c;
     return GQS_OK;
@@ -421,10 +422,66 @@ gqs_status gqs_sort_device(gqs_ctx* c, int dtype, void* d, void* dn, size_t n, v
     if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_sort_device: bad dtype");
     if (n < 2) return GQS_OK;
     if (!d || !dn) return c->set_error(GQS_EINVAL, "gqs_sort_device: null buffer");
-    if (n >= 0xFFFFFFFFull) return c->set_error(GQS_EINVAL, "gqs_sort_device: n must be < 2^32");
     if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+    if (n >= gqs::kMaxRunKeys || (c->large_run && n > c->large_run))
+        return c->sort_large(dtype, d, dn, n, c->large_run ? c->large_run : gqs::kLargeRunKeys, (hipStream_t)stream);
     return c->run(c->ws, dtype, d, dn, n, nullptr, 0, nullptr, (hipStream_t)stream, false);
 }
+
+gqs_status gqs_set_large_run(gqs_ctx* c, size_t keys) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (keys != 0 && (keys < 2 || keys >= gqs::kMaxRunKeys))
+        return c->set_error(GQS_EINVAL, "gqs_set_large_run: run length out of range");
+    c->large_run = keys;
+    return GQS_OK;
+}
+
+namespace gqs {
+
+// Runs of `run_keys` keys are sorted in place (dn's matching range is their
+// scratch); then rounds of pairwise merges double the run length, each round
+// reading one of d/dn and writing the other, with a lone last run copied
+// across; a final copy brings the keys back to d after an odd number of rounds.
+gqs_status Ctx::sort_large(int dtype, void* d, void* dn, uint64_t n, uint64_t run_keys, hipStream_t st) {
+    const size_t kb = key_bytes(dtype);
+    char* const d0 = static_cast<char*>(d);
+    char* const dn0 = static_cast<char*>(dn);
+    for (uint64_t off = 0; off < n; off += run_keys) {
+        const uint64_t m = std::min(run_keys, n - off);
+        if (m < 2) continue;
+        if (gqs_status s = run(ws, dtype, d0 + off * kb, dn0 + off * kb, m, nullptr, 0, nullptr, st, false)) return s;
+    }
+    const uint64_t need = merge_split_entries(dtype, n);
+    if (need > merge_split_cap) {
+        hipFree(merge_split);
+        merge_split = nullptr;
+        merge_split_cap = 0;
+        if (hipMalloc(&merge_split, need * sizeof(uint64_t)) != hipSuccess)
+            return set_error(GQS_ENOMEM, "sort_large: merge scratch");
+        merge_split_cap = need;
+    }
+    char* src = d0;
+    char* dst = dn0;
+    for (uint64_t w = run_keys; w < n; w *= 2) {
+        for (uint64_t a = 0; a < n; a += 2 * w) {
+            const uint64_t na = std::min(w, n - a), nb = std::min(w, n - a - na);
+            if (nb == 0) {
+                if (fail(hipMemcpyAsync(dst + a * kb, src + a * kb, na * kb, hipMemcpyDeviceToDevice, st), "merge copy"))
+                    return GQS_EDEVICE;
+            } else if (fail(launch_merge(dtype, src + a * kb, na, src + (a + na) * kb, nb, dst + a * kb, merge_split, st),
+                            "merge")) {
+                return GQS_EDEVICE;
+            }
+        }
+        std::swap(src, dst);
+    }
+    if (src != d0 && fail(hipMemcpyAsync(d0, src, n * kb, hipMemcpyDeviceToDevice, st), "merge copy"))
+        return GQS_EDEVICE;
+    return GQS_OK;
+}
+
+}  // namespace gqs
 
 gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
                               size_t total_keys, void* stream) {
@@ -434,6 +491,8 @@ gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const ui
     if (nseg == 0 || total_keys == 0) return GQS_OK;
     if (!d || !dn || !offsets) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: null buffer");
     if (nseg >= 0xFFFFFFFFull) return c->set_error(GQS_EINVAL, "gqs_sort_segmented: too many arrays");
+    if (total_keys >= gqs::kMaxRunKeys)
+        return c->set_error(GQS_EINVAL, "gqs_sort_segmented: total_keys must be < 2^32 - 2^16");
     if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
     return c->run(c->ws, dtype, d, dn, total_keys, offsets, (uint32_t)nseg, nullptr, (hipStream_t)stream, false);
 }
@@ -591,7 +650,8 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* c, int dtype, void* d, void* dn, 
     *out = nullptr;
     std::lock_guard<std::mutex> lk(c->mu);
     if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_graph_create_segmented: bad dtype");
-    if (!d || !dn || !offsets || !nseg_dev || max_nseg == 0 || max_total_keys < 2 || max_nseg >= 0xFFFFFFFFull)
+    if (!d || !dn || !offsets || !nseg_dev || max_nseg == 0 || max_total_keys < 2 || max_nseg >= 0xFFFFFFFFull ||
+        max_total_keys >= gqs::kMaxRunKeys)
         return c->set_error(GQS_EINVAL, "gqs_graph_create_segmented: bad argument");
     if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
     auto* g = new (std::nothrow) gqs_graph();

@@ -49,6 +49,13 @@ inline int step_kind(int step) {
 }
 hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st);
 
+// One run of the partition levels holds fewer than kMaxRunKeys keys: segment
+// sizes and in-segment offsets are 32-bit, and tile bounds (an offset plus
+// up to two 16 Ki-key tiles) must not wrap.  Larger sorts merge runs of
+// kLargeRunKeys keys.
+constexpr uint64_t kMaxRunKeys = (1ull << 32) - (1ull << 16);
+constexpr uint64_t kLargeRunKeys = 1ull << 31;
+
 Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
 uint32_t levels_needed(uint64_t n, const gqs_params& p);
 
@@ -98,6 +105,12 @@ struct Ctx {
     gqs_status run(Workspace& w, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
                    uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing);
     gqs_status fetch_stats();
+    // Sorts beyond one run's 32-bit sizes: runs of `run_keys` keys sorted by
+    // run(), then merged pairwise (merge scratch grown on demand).
+    uint64_t large_run = 0;  // 0: default
+    uint64_t* merge_split = nullptr;
+    size_t merge_split_cap = 0;
+    gqs_status sort_large(int dtype, void* d, void* dn, uint64_t n, uint64_t run_keys, hipStream_t st);
 };
 
 }  // namespace gqs

@@ -77,4 +77,11 @@ hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const
                                 LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max,
                                 hipStream_t st);
 
+// Merge of two sorted runs a[0, na) and b[0, nb) into out[0, na + nb) (out
+// overlaps neither run).  `split` holds merge_split_entries(dtype, na + nb)
+// words of scratch.
+hipError_t launch_merge(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, void* out, uint64_t* split,
+                        hipStream_t st);
+uint64_t merge_split_entries(int dtype, uint64_t n);
+
 }  // namespace gqs

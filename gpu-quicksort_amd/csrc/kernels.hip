@@ -47,33 +47,9 @@
 
 #include "internal.h"
 #include "kernels.h"
+#include "keys.h"
 
 namespace gqs {
-
-// ------------------------------------------------------------ key traits
-template <int DT> struct KeyOf { using K = uint32_t; };
-template <> struct KeyOf<DT_F64> { using K = uint64_t; };
-
-template <int DT>
-__device__ __forceinline__ typename KeyOf<DT>::K to_key(typename KeyOf<DT>::K u) {
-    if constexpr (DT == DT_F32) return (u >> 31) ? ~u : (u | 0x80000000u);
-    else if constexpr (DT == DT_F64) return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-    else return u;
-}
-template <int DT>
-__device__ __forceinline__ typename KeyOf<DT>::K from_key(typename KeyOf<DT>::K k) {
-    if constexpr (DT == DT_F32) return (k >> 31) ? (k & 0x7FFFFFFFu) : ~k;
-    else if constexpr (DT == DT_F64) return (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
-    else return k;
-}
-template <class K>
-__device__ __forceinline__ K key_max() { return ~K(0); }
-
-template <int DT>
-__device__ __forceinline__ typename KeyOf<DT>::K load_key(const typename KeyOf<DT>::K* p, uint64_t i, bool raw) {
-    typename KeyOf<DT>::K v = p[i];
-    return raw ? to_key<DT>(v) : v;
-}
 
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
@@ -539,7 +515,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             ncand = 2;
         } else {
             uint32_t k = 2;
-            const uint32_t want = (n + a.leaf_target - 1) / a.leaf_target;
+            const uint32_t want = (uint32_t)(((uint64_t)n + a.leaf_target - 1) / a.leaf_target);
             while (k < want && k < a.max_fanout) k <<= 1;
             uint32_t S = 1;
             while (S < k * a.oversample && S < (uint32_t)kSampleMax) S <<= 1;
@@ -636,7 +612,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             while ((1u << ld) < w) ++ld;
             SegPlan p;
             p.tile_begin = 0;
-            p.ntiles = (n + TileShape<K>::kTileKeys - 1) / TileShape<K>::kTileKeys;
+            p.ntiles = (uint32_t)(((uint64_t)n + TileShape<K>::kTileKeys - 1) / TileShape<K>::kTileKeys);
             p.mat_base = 0;
             p.nbuckets = nb;
             p.bucket_base = 0;

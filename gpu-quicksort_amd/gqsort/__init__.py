@@ -82,6 +82,7 @@ _SIGS = {
     "gqs_default_params": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params)]),
     "gqs_sort": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     "gqs_sort_device": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
+    "gqs_set_large_run": (ctypes.c_int, [_P, ctypes.c_size_t]),
     "gqs_sort_segmented": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t, _P]),
     "gqs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "gqs_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -242,6 +243,11 @@ class Context:
 
     def sort_device(self, dtype, d, dn, n, stream=None):
         _check(self._lib.gqs_sort_device(self._h, dtype, _ptr(d), _ptr(dn), n, _stream(stream)), self)
+
+    def set_large_run(self, keys=0):
+        """Sorts above `keys` keys run as sorted runs of `keys` keys merged
+        pairwise (0: only n >= 2^32 - 2^16, with runs of 2^31 keys)."""
+        _check(self._lib.gqs_set_large_run(self._h, keys), self)
 
     def sort_segmented(self, dtype, d, dn, offsets, nseg, total, stream=None):
         _check(self._lib.gqs_sort_segmented(self._h, dtype, _ptr(d), _ptr(dn), _ptr(offsets), nseg, total,

@@ -92,12 +92,26 @@ gqs_status gqs_default_params(int dtype, gqs_params* out);
 gqs_status gqs_sort(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n);
 
 /* Device-resident: d, dn are device pointers; stream-ordered, returns after
- * enqueueing (no host synchronisation unless params.host_driven). */
+ * enqueueing (no host synchronisation unless params.host_driven).
+ *
+ * n is a size_t on both entry points.  The reference caps n at 32 bits
+ * (CL12/QuicksortMain.cpp:405; uint indices in its records,
+ * CL12/Quicksort.h:105-145); here n >= 2^32 - 2^16 is sorted as runs of 2^31 keys
+ * (each through the partition levels) merged pairwise with 64-bit indices,
+ * ping-ponging between d and dn.  gqs_get_stats then describes the last run. */
 gqs_status gqs_sort_device(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n, void* stream);
+
+/* Run length of the large-n path: with keys > 0, every sort of more than
+ * `keys` keys on this context takes that path with runs of `keys` keys
+ * (2 <= keys < 2^32 - 2^16); keys == 0 restores the default (runs of 2^31
+ * keys, only for n >= 2^32 - 2^16).  No counterpart in the reference: it lets tests
+ * and tuning exercise the run-and-merge path at sizes a check can afford. */
+gqs_status gqs_set_large_run(gqs_ctx* ctx, size_t keys);
 
 /* Segmented: nseg independent arrays packed in d, array i occupying
  * [offsets[i], offsets[i+1]) (offsets: nseg+1 device uint64).  Each array is
- * sorted on its own; one chain of launches for all of them. */
+ * sorted on its own; one chain of launches for all of them.  total_keys (and a
+ * graph's max_total_keys) must be below 2^32 - 2^16. */
 gqs_status gqs_sort_segmented(gqs_ctx* ctx, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
                               size_t total_keys, void* stream);
 

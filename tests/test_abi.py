@@ -51,6 +51,7 @@ def test_null_arguments_are_rejected():
     lib = gqsort.load()
     assert lib.gqs_sort(None, 0, None, None, 10) == 1
     assert lib.gqs_sort_device(None, 0, None, None, 10, None) == 1
+    assert lib.gqs_set_large_run(None, 1 << 20) == 1
     assert lib.gqs_default_params(0, None) == 1
     assert lib.gqs_generate(0, 0, 1, None, 10) == 1
     assert lib.gqs_generate(0, 99, 1, None, 0) == 0
