@@ -1634,9 +1634,11 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
             continue;
         }
         // 2. local buckets by interpolation between min and max, about
-        //    sort_threshold keys each; counted first, then every key takes
-        //    its slot from a per-bucket cursor (order inside a bucket is free:
-        //    the bucket is sorted next)
+        //    sort_threshold keys each.  Every key takes its rank inside its
+        //    bucket from a returning LDS add (order inside a bucket is free:
+        //    the bucket is sorted next) and keeps it with the bucket index as
+        //    b << 13 | rank (b < 512, rank < CAP <= 8192), so the placement
+        //    needs neither a second atomic nor the bucket again
         uint32_t nb = 2;
         while (nb < (uint32_t)kLeafLocal && nb * a.sort_threshold < n) nb <<= 1;
         const double scale = (double)nb / ((double)(mx - mn) + 1.0);
@@ -1646,33 +1648,36 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
             else b = (uint32_t)((double)(v - mn) * scale);
             return min(b, nb - 1);
         };
+        static_assert(KPT % 4 == 0, "rank adds go in fours");
+        static_assert(CAP <= 8192u && kLeafLocal <= 512, "bucket and rank share one word");
+        uint32_t br[KPT];
 #pragma unroll
-        for (int j = 0; j < KPT; ++j)
-            if (j * NT + tid < n) hist_add_asm(hist, local_bucket(kv[j]));
+        for (int j = 0; j < KPT; ++j) br[j] = 0;
+#pragma unroll
+        for (int j = 0; j < KPT; j += 4) {
+            if ((uint32_t)j * NT < n) {  // uniform: some lane has a key here
+                uint32_t ad[4], rk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    br[j + u] = local_bucket(kv[j + u]);
+                    ad[u] = (j + u) * NT + tid < n ? lds_addr(&hist[br[j + u]]) : lds_addr(&S.junk[lane]);
+                }
+                lds_add_rtn4(ad[0], ad[1], ad[2], ad[3], 1u, rk[0], rk[1], rk[2], rk[3]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) br[j + u] = (br[j + u] << 13) | rk[u];
+            }
+        }
         lds_sync();
-        // 3. bucket starts: thread t owns local bucket t; hist becomes the cursor
+        // 3. bucket starts: thread t owns local bucket t
         const uint32_t mycnt = (tid < nb) ? hist[tid] : 0u;
         uint32_t tot;
         const uint32_t mystart = block_excl_scan<NT, true>(mycnt, scanw, &tot);
-        if (tid < nb) {
-            bst[tid] = mystart;
-            hist[tid] = mystart;
-        }
+        if (tid < nb) bst[tid] = mystart;
         lds_sync();
         // 4. place keys by bucket
-        static_assert(KPT % 4 == 0, "cursor adds go in fours");
 #pragma unroll
-        for (int j = 0; j < KPT; j += 4) {
-            if ((uint32_t)j * NT >= n) break;  // no lane has a key from here on
-            uint32_t ad[4], pos[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                ad[u] = (j + u) * NT + tid < n ? lds_addr(&hist[local_bucket(kv[j + u])]) : lds_addr(&S.junk[lane]);
-            lds_add_rtn4(ad[0], ad[1], ad[2], ad[3], 1u, pos[0], pos[1], pos[2], pos[3]);
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if ((j + u) * NT + tid < n) A[pos[u]] = kv[j + u];
-        }
+        for (int j = 0; j < KPT; ++j)
+            if (j * NT + tid < n) A[bst[br[j] >> 13] + (br[j] & 0x1FFFu)] = kv[j];
         lds_sync();
         // 5. each thread sorts its bucket in registers with an odd-even merge
         //    network sized to the largest bucket of its wave
