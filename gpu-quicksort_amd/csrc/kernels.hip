@@ -807,21 +807,23 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
         const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
         const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
         const K* base = src + seg.start;
+        // Loads are unconditional (index clamped to the tile's last key, the
+        // value masked afterwards): loads under a branch cannot be counted,
+        // and the compiler would then wait for the next sub-tile's prefetch
+        // before classifying the current one.
+        auto load = [&](uint32_t idx) {
+            const K v = load_key<DT>(base, min(idx, hi - 1u), raw);
+            return idx < hi ? v : K(0);
+        };
         K keys[TS::kKeysPerThread];
 #pragma unroll
-        for (int j = 0; j < TS::kKeysPerThread; ++j) {
-            const uint32_t idx = lo + j * kPartWG + tid;
-            keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
-        }
+        for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = load(lo + j * kPartWG + tid);
         for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
             // prefetch the next sub-tile while this one is classified
             K nxt[TS::kKeysPerThread];
             const uint32_t nsub = sub + TS::kSubKeys;
 #pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) {
-                const uint32_t idx = nsub + j * kPartWG + tid;
-                nxt[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
-            }
+            for (int j = 0; j < TS::kKeysPerThread; ++j) nxt[j] = load(nsub + j * kPartWG + tid);
             uint32_t bk[TS::kKeysPerThread];
             classify_lut<K, TS::kKeysPerThread>(keys, bk, L, p);
             {
