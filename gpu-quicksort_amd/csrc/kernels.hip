@@ -1589,11 +1589,13 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
             for (int j = 0; j < KPT; ++j) kv[j] = P[j * NT + tid];
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
-                const bool in = j * NT + tid < cur.n;
-                const K v = cur.raw ? to_key<DT>(kv[j]) : kv[j];
-                kv[j] = in ? v : key_max<K>();
-                mn = kmin(mn, kv[j]);
-                mx = kmax(mx, in ? v : K(0));
+                if ((uint32_t)j * NT < cur.n) {  // uniform; later steps skip these slots too
+                    const bool in = j * NT + tid < cur.n;
+                    const K v = cur.raw ? to_key<DT>(kv[j]) : kv[j];
+                    kv[j] = in ? v : key_max<K>();
+                    mn = kmin(mn, kv[j]);
+                    mx = kmax(mx, in ? v : K(0));
+                }
             }
             wave_minmax<K, 32>(mn, mx);
             if (lane == 0) {
