@@ -976,6 +976,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     constexpr int KPT = TS::kKeysPerThread;
     constexpr uint32_t kSub = (uint32_t)kScatterWG * KPT;  // keys per staged sub-tile
     static_assert(TS::kTileKeys % kSub == 0, "a tile is whole sub-tiles");
+    constexpr bool kEarlyPrefetch = DT == DT_U32;  // keys need no transform
     __shared__ TileLds<K> L;
     __shared__ uint32_t lstart[kSplStride];
     __shared__ uint32_t delta[kSplStride];  // output offset - staging offset, per bucket
@@ -1036,26 +1037,48 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
         const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
         const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
         const K* base = src + seg.start;
+        // Key loads are unconditional (index clamped into the range, value
+        // masked afterwards) and the float key transform is applied only
+        // when the keys are used: a transform next to a load under a branch
+        // made the compiler wait for every prefetch load on the spot.
         if (!have_keys) {
+            K raw_keys[KPT];
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) raw_keys[j] = base[min(lo + j * kScatterWG + tid, hi - 1u)];
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
-                const uint32_t idx = lo + j * kScatterWG + tid;
-                keys[j] = (idx < hi) ? load_key<DT>(base, idx, raw) : K(0);
+                const K v = raw ? to_key<DT>(raw_keys[j]) : raw_keys[j];
+                keys[j] = (lo + j * kScatterWG + tid < hi) ? v : K(0);
             }
         }
         have_keys = false;
         bool anyd = false;
         for (uint32_t sub = lo; sub < hi; sub += kSub) {
             const bool first = sub == lo;
-            uint32_t bk[KPT];
-            uint32_t rk[KPT];
-            bool valid[KPT];
-            classify_lut<K, KPT>(keys, bk, L, p);
+            // bucket (< 256) and rank inside the sub-tile (< kSub) of each key,
+            // packed as bucket << 16 | rank to keep the registers free for the
+            // prefetched keys
+            static_assert(kSub <= 65536 && kSplStride <= 256, "bucket and rank share one word");
+            uint32_t br[KPT];
+            {
+                uint32_t bk[KPT];
+                uint32_t rk[KPT];
+                bool valid[KPT];
+                classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) valid[j] = sub + j * kScatterWG + tid < hi;
-            hist_rank_n<KPT>(L.hist, bk, valid, rk);
+                for (int j = 0; j < KPT; ++j) valid[j] = sub + j * kScatterWG + tid < hi;
+                hist_rank_n<KPT>(L.hist, bk, valid, rk);
+#pragma unroll
+                for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
+            }
             // prefetch: the next sub-tile, or the next tile's first sub-tile
-            // when that tile is this work-group's and in the same segment
+            // when that tile is this work-group's and in the same segment.
+            // uint32 keys are loaded here, before the scan and the staging.
+            // Float keys are loaded once the current keys are staged (both
+            // sets live at once, with the transform pending, spill), as
+            // unconditional clamped loads whose transform and mask wait until
+            // the keys are used: a transform next to a load under a branch
+            // made the compiler wait for every prefetch load on the spot.
             uint32_t nsub = sub + kSub, nhi = hi;
             bool next_tile = false;
             if (nsub >= hi) {
@@ -1064,10 +1087,12 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
                 nhi = next_tile ? min(nsub + (uint32_t)TS::kTileKeys, seg.n) : 0u;
             }
             K nxt[KPT];
+            if constexpr (kEarlyPrefetch) {
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const uint32_t idx = nsub + j * kScatterWG + tid;
-                nxt[j] = (idx < nhi) ? load_key<DT>(base, idx, raw) : K(0);
+                for (int j = 0; j < KPT; ++j) {
+                    const uint32_t idx = nsub + j * kScatterWG + tid;
+                    nxt[j] = (idx < nhi) ? base[idx] : K(0);
+                }
             }
             if (first && tid < p.nbuckets) {
                 const bool dir = (bsize & kBucketDirect) != 0;
@@ -1090,13 +1115,23 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
             for (int j0 = 0; j0 < KPT; j0 += KPT / 2) {
                 uint32_t ps[KPT / 2];
 #pragma unroll
-                for (int j = 0; j < KPT / 2; ++j) ps[j] = lstart[bk[j0 + j]];
+                for (int j = 0; j < KPT / 2; ++j) ps[j] = lstart[br[j0 + j] >> 16];
 #pragma unroll
                 for (int j = 0; j < KPT / 2; ++j) {
-                    if (valid[j0 + j]) {
-                        stage[ps[j] + rk[j0 + j]] = keys[j0 + j];
-                        sbk[ps[j] + rk[j0 + j]] = (uint8_t)bk[j0 + j];
+                    if (sub + (j0 + j) * kScatterWG + tid < hi) {
+                        const uint32_t pos = ps[j] + (br[j0 + j] & 0xFFFFu);
+                        stage[pos] = keys[j0 + j];
+                        sbk[pos] = (uint8_t)(br[j0 + j] >> 16);
                     }
+                }
+            }
+            if constexpr (!kEarlyPrefetch) {
+                if (nsub < nhi) {  // uniform
+#pragma unroll
+                    for (int j = 0; j < KPT; ++j) nxt[j] = base[min(nsub + j * kScatterWG + tid, nhi - 1u)];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
                 }
             }
             __syncthreads();
@@ -1128,8 +1163,16 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
                 }
             }
             my_cursor += hv;
+            if constexpr (kEarlyPrefetch) {
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+                for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < KPT; ++j) {
+                    const K v = raw ? to_key<DT>(nxt[j]) : nxt[j];
+                    keys[j] = (nsub + j * kScatterWG + tid < nhi) ? v : K(0);
+                }
+            }
             have_keys = next_tile;
         }
     }
