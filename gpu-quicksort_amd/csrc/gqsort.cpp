@@ -91,17 +91,61 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     return c;
 }
 
+// Parameters a sort of n keys of this type runs with: the tuned (or set)
+// parameters, the leaf size capped by what one work-group holds in LDS.
+gqs_params Ctx::run_params(int dtype, uint64_t n) const {
+    gqs_params p = effective_params(dtype, n);
+    p.leaf_max = std::min(p.leaf_max, dtype == DT_F64 ? 4096u : 8192u);
+    return p;
+}
+
+// Capacities for any sort of at most n keys (and nseg_in arrays) of this
+// type, whichever size class's parameters it runs with: capacities grow with
+// n, so the maximum over the parameter sets at n bounds every smaller sort.
+Caps Ctx::caps_upto(uint64_t n, uint32_t nseg_in, int dtype) const {
+    auto widen = [](Caps& a, const Caps& b) {
+        a.seg = std::max(a.seg, b.seg);
+        a.mat = std::max(a.mat, b.mat);
+        a.bkt = std::max(a.bkt, b.bkt);
+        a.part = std::max(a.part, b.part);
+        a.leaves = std::max(a.leaves, b.leaves);
+        a.fill = std::max(a.fill, b.fill);
+    };
+    const uint32_t cap = dtype == DT_F64 ? 4096u : 8192u;
+    gqs_params p = params;
+    p.leaf_max = std::min(p.leaf_max, cap);
+    Caps c = caps_for(n, nseg_in, p, dtype);
+    widen(c, caps_for(n, nseg_in, run_params(dtype, n), dtype));
+    if (use_tuned)
+        for (const auto& kv : tuned)
+            if (kv.first.first == dtype) {
+                gqs_params q = kv.second;
+                q.leaf_max = std::min(q.leaf_max, cap);
+                widen(c, caps_for(n, nseg_in, q, dtype));
+            }
+    return c;
+}
+
 gqs_status Ctx::ensure_workspace(Workspace& ws, uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
-    const Caps need = caps_for(n, nseg_in, p, dtype);
+    return ensure_caps(ws, caps_for(n, nseg_in, p, dtype), dtype);
+}
+
+gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
     const size_t kb = key_bytes(dtype);
     if (ws.ctl && need.seg <= ws.cap.seg && need.mat <= ws.cap.mat && need.bkt <= ws.cap.bkt &&
         need.part <= ws.cap.part && need.leaves <= ws.cap.leaves && need.fill <= ws.cap.fill && kb <= ws.key_bytes)
         return GQS_OK;
-    if (ws.ctl) {
+    Caps c = need;
+    if (ws.ctl) {  // grow: keep every capacity the workspace already had
+        c.seg = std::max(c.seg, ws.cap.seg);
+        c.mat = std::max(c.mat, ws.cap.mat);
+        c.bkt = std::max(c.bkt, ws.cap.bkt);
+        c.part = std::max(c.part, ws.cap.part);
+        c.leaves = std::max(c.leaves, ws.cap.leaves);
+        c.fill = std::max(c.fill, ws.cap.fill);
         hipStreamSynchronize(last_stream);
         ws.release();
     }
-    Caps c = need;
     // Grow with headroom so that nearby sizes reuse the workspace.
     c.seg += c.seg / 4;
     c.mat += c.mat / 4;
@@ -191,11 +235,8 @@ uint32_t levels_needed(uint64_t n, const gqs_params& p) {
 // [0, n) of d (offsets == nullptr) or nseg arrays delimited by offsets.
 gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
                     uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing) {
-    gqs_params p = effective_params(dtype, n);
+    gqs_params p = run_params(dtype, n);
     if (capturing) p.host_driven = 0;
-    // a work-group leaf holds 32 (u32/f32) or 16 (f64) keys per thread
-    const uint32_t cap = (dtype == DT_F64) ? 4096u : 8192u;
-    p.leaf_max = std::min(p.leaf_max, cap);
     if (gqs_status s = ensure_workspace(ws, n, nseg_in, p, dtype)) return s;
     if (!capturing) last_stream = st;
     capturing_now = capturing;
@@ -657,9 +698,9 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* c, int dtype, void* d, void* dn, 
     auto* g = new (std::nothrow) gqs_graph();
     if (!g) return GQS_ENOMEM;
     g->ctx = c;
-    gqs_params p = c->params;
-    p.leaf_max = std::min(p.leaf_max, dtype == DT_F64 ? 4096u : 8192u);
-    if (gqs_status s = c->ensure_workspace(g->ws, max_total_keys, (uint32_t)max_nseg, p, dtype)) {
+    // sized for the parameters the capture runs with (the size class of
+    // max_total_keys): no allocation may happen while the stream captures
+    if (gqs_status s = c->ensure_caps(g->ws, c->caps_upto(max_total_keys, (uint32_t)max_nseg, dtype), dtype)) {
         delete g;
         return s;
     }

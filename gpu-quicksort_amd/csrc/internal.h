@@ -93,6 +93,9 @@ struct Ctx {
     gqs_status set_error(gqs_status s, const char* msg);
     bool capturing_now = false;
     gqs_status ensure_workspace(Workspace& w, uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
+    gqs_status ensure_caps(Workspace& w, const Caps& need, int dtype);
+    gqs_params run_params(int dtype, uint64_t n) const;
+    Caps caps_upto(uint64_t n, uint32_t nseg_in, int dtype) const;
     Grid grid(const gqs_params& p) const;
     gqs_params effective_params(int dtype, uint64_t n) const;
     std::map<std::pair<int, int>, gqs_params> tuned;  // (dtype, floor(log2 n)) -> parameters

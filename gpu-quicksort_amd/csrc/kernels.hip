@@ -1399,15 +1399,28 @@ __device__ __forceinline__ void wave_sort_batch(const typename KeyOf<DT>::K* A, 
         }
 }
 
-// One wave sorts n <= 64*E keys read through `ld(i)` and stores key i with `st(i, key)`.
-template <class K, int E, class LD, class ST>
+// One wave sorts n <= 64*E keys read through `ld(i)` and stores key i with
+// `st(i, key)`.  With kClamp every lane reads all E slots (index clamped to
+// n - 1, value masked afterwards): global reads under a branch, followed by
+// ld's key transform, would be waited for one by one.  Without it (LDS reads,
+// which need the registers less) slots past n are not read.
+template <class K, int E, bool kClamp, class LD, class ST>
 __device__ __forceinline__ void wave_sort_n(uint32_t n, LD ld, ST st) {
+    if (n == 0) return;
     const uint32_t lane = lane_id();
     K v[E];
+    if constexpr (kClamp) {
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-        const uint32_t i = lane * E + r;
-        v[r] = (i < n) ? ld(i) : key_max<K>();
+        for (int r = 0; r < E; ++r) v[r] = ld(min(lane * E + r, n - 1u));
+#pragma unroll
+        for (int r = 0; r < E; ++r)
+            if (lane * E + r >= n) v[r] = key_max<K>();
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const uint32_t i = lane * E + r;
+            v[r] = (i < n) ? ld(i) : key_max<K>();
+        }
     }
     wave_bitonic<K, E>(v);
 #pragma unroll
@@ -1417,12 +1430,12 @@ __device__ __forceinline__ void wave_sort_n(uint32_t n, LD ld, ST st) {
     }
 }
 
-template <class K, class LD, class ST>
+template <class K, bool kClamp = false, class LD, class ST>
 __device__ __forceinline__ void wave_sort_any(uint32_t n, LD ld, ST st) {
-    if (n <= 64) wave_sort_n<K, 1>(n, ld, st);
-    else if (n <= 128) wave_sort_n<K, 2>(n, ld, st);
-    else if (n <= 256) wave_sort_n<K, 4>(n, ld, st);
-    else wave_sort_n<K, 8>(n, ld, st);
+    if (n <= 64) wave_sort_n<K, 1, kClamp>(n, ld, st);
+    else if (n <= 128) wave_sort_n<K, 2, kClamp>(n, ld, st);
+    else if (n <= 256) wave_sort_n<K, 4, kClamp>(n, ld, st);
+    else wave_sort_n<K, 8, kClamp>(n, ld, st);
 }
 
 struct LeafDesc {
@@ -1490,7 +1503,7 @@ __global__ __launch_bounds__(kLeafWG) void k_leaf_small(LeafArgs a) {
             const K* src = (ds.from_dn ? (const K*)a.dn : (const K*)a.d) + ds.start;
             K* out = dout + ds.start;
             const bool raw = ds.raw;
-            wave_sort_any<K>(
+            wave_sort_any<K, true>(
                 ds.n, [&](uint32_t i) { return load_key<DT>(src, i, raw); },
                 [&](uint32_t i, K v) { out[i] = from_key<DT>(v); });
         }

@@ -232,7 +232,6 @@ gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_serve
     s->cfg = c;
     s->slots.resize(c.slots);
     hipError_t e = hipSetDevice(device);
-    gqs_params p = s->ctx->params;
     for (auto& sl : s->slots) {
         if (!e) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (!e) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
@@ -242,11 +241,12 @@ gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_serve
         if (!e) e = hipHostMalloc(reinterpret_cast<void**>(&sl.offs_host), (c.max_batch_requests + 1) * 8,
                                   hipHostMallocDefault);
         if (e) break;
-        for (int dt : {GQS_U32, GQS_F64}) {
-            gqs_params q = p;
-            q.leaf_max = std::min(q.leaf_max, dt == GQS_F64 ? 4096u : 8192u);
+        // one workspace per slot, big enough for a batch of any key type and
+        // the parameters of any size class it may run with
+        for (int dt : {GQS_U32, GQS_F32, GQS_F64}) {
             const uint64_t keys = dt == GQS_F64 ? c.slot_keys : c.slot_keys * 2;
-            if (s->ctx->ensure_workspace(sl.ws, keys, c.max_batch_requests, q, dt) != GQS_OK) e = hipErrorOutOfMemory;
+            if (s->ctx->ensure_caps(sl.ws, s->ctx->caps_upto(keys, c.max_batch_requests, dt), dt) != GQS_OK)
+                e = hipErrorOutOfMemory;
         }
         if (e) break;
     }

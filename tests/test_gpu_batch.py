@@ -97,3 +97,40 @@ def test_graph_replays(gpu_ctx):
     finally:
         g.close()
         stream.close()
+
+
+@pytest.mark.parametrize("dtype", [gqsort.U32, gqsort.F32, gqsort.F64])
+def test_graph_capture_in_fresh_tuned_context(dtype):
+    """A graph captured on a new context that uses the shipped tuned table:
+    its workspace is sized for the parameters the capture runs with (the
+    tuned class of max_total_keys, which can differ from the context's own),
+    so nothing is allocated while the stream captures.  Each graph is replayed
+    once and checked against std::sort."""
+    ctx = gqsort.Context(0)
+    try:
+        kb = gqsort.KEY_BYTES[dtype]
+        for max_total in (1 << 16, 1 << 20, 3 << 20):
+            max_nseg = 64
+            d = gqsort.DeviceBuffer(ctx, max_total * kb)
+            dn = gqsort.DeviceBuffer(ctx, max_total * kb)
+            od = gqsort.DeviceBuffer(ctx, (max_nseg + 1) * 8)
+            nd = gqsort.DeviceBuffer(ctx, 4)
+            stream = gqsort.Stream(ctx)
+            print(f"case fresh-context graph dtype={dtype} max_total={max_total}", flush=True)
+            g = gqsort.SegmentedGraph(ctx, dtype, d, dn, od, nd, max_nseg, max_total)
+            try:
+                lens = gqsort.batch_lengths(max_nseg, 16, max_total // 8, seed=7, total=max_total * 0.9)
+                keys, offs = make_batch(dtype, lens, 11)
+                d.upload(keys, stream)
+                od.upload(offs, stream)
+                nd.upload(np.array([max_nseg], np.uint32), stream)
+                g.launch(stream)
+                stream.sync()
+                check_batch(d.download(keys.dtype, keys.size, stream), keys, offs, f"fresh graph {max_total}")
+            finally:
+                g.close()
+                stream.close()
+                for b in (d, dn, od, nd):
+                    b.free()
+    finally:
+        ctx.close()
