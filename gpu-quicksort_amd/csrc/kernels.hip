@@ -842,42 +842,82 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
 }
 
 // ------------------------------------------------------------ scan
+// Exclusive scan of the level's count matrix (mat_size entries) in chunks of
+// kScanChunk: k_scan_reduce sums each chunk, k_scan_apply adds the sums of the
+// chunks before its own and scans it.  Thread t holds entries [16t, 16t + 16)
+// of a chunk, moved as four 16-byte loads and stores (the matrix is
+// 256-byte aligned and chunks start at multiples of 16 KiB).
+constexpr int kScanPer = kScanChunk / 256;
+static_assert(kScanPer % 4 == 0, "whole 16-byte vectors per thread");
+
+__device__ __forceinline__ void scan_load(const uint32_t* mat, uint32_t lo, uint32_t size, uint32_t (&v)[kScanPer]) {
+    if (lo + kScanPer <= size) {
+        const uint4* q = reinterpret_cast<const uint4*>(mat + lo);
+#pragma unroll
+        for (int j = 0; j < kScanPer / 4; ++j) {
+            const uint4 x = q[j];
+            v[4 * j] = x.x;
+            v[4 * j + 1] = x.y;
+            v[4 * j + 2] = x.z;
+            v[4 * j + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) v[j] = (lo + j < size) ? mat[lo + j] : 0u;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_scan_reduce(LevelArgs a) {
     __shared__ uint32_t wsum[4];
     const uint32_t nch = a.ctl->nchunks, size = a.ctl->mat_size;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        const uint32_t lo = c * kScanChunk;
-        const uint32_t hi = min(lo + (uint32_t)kScanChunk, size);
-        uint32_t v = 0;
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) v += a.mat[i];
-        v = block_sum<256>(v, wsum);
-        if (threadIdx.x == 0) a.part[c] = v;
+        uint32_t v[kScanPer];
+        scan_load(a.mat, c * kScanChunk + threadIdx.x * kScanPer, size, v);
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) sum += v[j];
+        sum = block_sum<256>(sum, wsum);
+        if (threadIdx.x == 0) a.part[c] = sum;
         __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(256) void k_scan_apply(LevelArgs a) {
     __shared__ uint32_t wsum[4];
-    constexpr int PER = kScanChunk / 256;
     const uint32_t nch = a.ctl->nchunks, size = a.ctl->mat_size;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        // sums of chunks [0, c): four independent loads per thread per round
         uint32_t pre = 0;
-        for (uint32_t i = threadIdx.x; i < c; i += 256) pre += a.part[i];
+        for (uint32_t i = threadIdx.x; i < c; i += 4 * 256) {
+            uint32_t x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = a.part[min(i + u * 256u, c - 1u)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pre += (i + u * 256u < c) ? x[u] : 0u;
+        }
         pre = block_sum<256>(pre, wsum);
-        const uint32_t lo = c * kScanChunk + threadIdx.x * PER;
-        uint32_t v[PER];
+        const uint32_t lo = c * kScanChunk + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer];
+        scan_load(a.mat, lo, size, v);
         uint32_t sum = 0;
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            v[j] = (lo + j < size) ? a.mat[lo + j] : 0u;
-            sum += v[j];
-        }
+        for (int j = 0; j < kScanPer; ++j) sum += v[j];
         uint32_t tot;
         uint32_t run = pre + block_excl_scan<256>(sum, wsum, &tot);
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            if (lo + j < size) a.mat[lo + j] = run;
-            run += v[j];
+        for (int j = 0; j < kScanPer; ++j) {
+            const uint32_t x = v[j];
+            v[j] = run;
+            run += x;
+        }
+        if (lo + kScanPer <= size) {
+            uint4* q = reinterpret_cast<uint4*>(a.mat + lo);
+#pragma unroll
+            for (int j = 0; j < kScanPer / 4; ++j) q[j] = make_uint4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanPer; ++j)
+                if (lo + j < size) a.mat[lo + j] = v[j];
         }
         __syncthreads();
     }
