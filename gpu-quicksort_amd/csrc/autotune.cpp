@@ -93,7 +93,7 @@ gqs_status gqs_autotune(gqs_ctx* c, int dtype, size_t n, int gen, uint32_t reps,
     };
     gqs_params cur = saved;
     cur.host_driven = 0;
-    const uint32_t lmax = dtype == GQS_F64 ? 4096u : 8192u;
+    const uint32_t lmax = 2 * gqs::leaf_cap(dtype);
     cur.leaf_max = std::min(cur.leaf_max, lmax);
     double best_ms = 0;
     if (status == GQS_OK) status = time_params(cur, &best_ms);

@@ -56,6 +56,7 @@ gqs_status Ctx::set_error(gqs_status s, const char* msg) {
 }
 
 void Workspace::release() {
+    hipFree(huge);
     hipFree(segs[0]);
     hipFree(segs[1]);
     hipFree(plans);
@@ -86,6 +87,9 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     c.mat = 256 * (n / tile + 1) + sum_nb + 256;
     c.part = c.mat / kScanChunk + 2;
     c.leaves = std::max<uint64_t>(1, nseg_in);
+    // large leaves per level (or per segmented batch): each above one
+    // 512-thread work-group's capacity
+    c.huge = n / ((uint64_t)leaf_cap(dtype) + 1) + 2;
     // fill tasks per level: ceil(size/chunk) per equality bucket
     c.fill = n / kFillChunk + c.bkt + 1;
     return c;
@@ -95,7 +99,7 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
 // parameters, the leaf size capped by what one work-group holds in LDS.
 gqs_params Ctx::run_params(int dtype, uint64_t n) const {
     gqs_params p = effective_params(dtype, n);
-    p.leaf_max = std::min(p.leaf_max, dtype == DT_F64 ? 4096u : 8192u);
+    p.leaf_max = std::min(p.leaf_max, 2 * leaf_cap(dtype));
     return p;
 }
 
@@ -110,8 +114,9 @@ Caps Ctx::caps_upto(uint64_t n, uint32_t nseg_in, int dtype) const {
         a.part = std::max(a.part, b.part);
         a.leaves = std::max(a.leaves, b.leaves);
         a.fill = std::max(a.fill, b.fill);
+        a.huge = std::max(a.huge, b.huge);
     };
-    const uint32_t cap = dtype == DT_F64 ? 4096u : 8192u;
+    const uint32_t cap = 2 * leaf_cap(dtype);
     gqs_params p = params;
     p.leaf_max = std::min(p.leaf_max, cap);
     Caps c = caps_for(n, nseg_in, p, dtype);
@@ -133,7 +138,8 @@ gqs_status Ctx::ensure_workspace(Workspace& ws, uint64_t n, uint32_t nseg_in, co
 gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
     const size_t kb = key_bytes(dtype);
     if (ws.ctl && need.seg <= ws.cap.seg && need.mat <= ws.cap.mat && need.bkt <= ws.cap.bkt &&
-        need.part <= ws.cap.part && need.leaves <= ws.cap.leaves && need.fill <= ws.cap.fill && kb <= ws.key_bytes)
+        need.part <= ws.cap.part && need.leaves <= ws.cap.leaves && need.fill <= ws.cap.fill &&
+        need.huge <= ws.cap.huge && kb <= ws.key_bytes)
         return GQS_OK;
     Caps c = need;
     if (ws.ctl) {  // grow: keep every capacity the workspace already had
@@ -143,6 +149,7 @@ gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
         c.part = std::max(c.part, ws.cap.part);
         c.leaves = std::max(c.leaves, ws.cap.leaves);
         c.fill = std::max(c.fill, ws.cap.fill);
+        c.huge = std::max(c.huge, ws.cap.huge);
         hipStreamSynchronize(last_stream);
         ws.release();
     }
@@ -165,6 +172,7 @@ gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
     if (!e) e = dalloc(&ws.fill, c.fill);
     if (!e) e = dalloc(&ws.ctl, (size_t)kMaxLevels + 1);
     if (!e) e = dalloc(&ws.leaves, c.leaves);
+    if (!e) e = dalloc(&ws.huge, c.huge);
     if (e) {
         ws.release();
         fail(e, "workspace allocation");
@@ -202,6 +210,7 @@ Grid Ctx::grid(const gqs_params& p) const {
     g.scan = cus * 2;
     g.plan = cus * 2;
     g.leaf = cus * (int)p.leaf_wgs_per_cu;
+    g.huge = cus;
     return g;
 }
 
@@ -255,16 +264,17 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         prof_begin(7, st);
         if (fail(launch_init(ws.ctl, kMaxLevels, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
         if (fail(launch_seed_segments(offsets, nseg_in, nseg_dev, ws.segs[0], &ws.ctl[0], ws.leaves,
-                                      &ws.ctl[0].nleaf_list, p.leaf_max, st),
+                                      &ws.ctl[0].nleaf_list, p.leaf_max, leaf_cap(dtype), ws.huge, st),
                  "seed"))
             return GQS_EDEVICE;
         prof_end(st);
     } else {
         const bool as_leaf = n <= p.leaf_max;
+        const bool as_huge = as_leaf && n > leaf_cap(dtype);
         L = as_leaf ? 0 : levels_needed(n, p);
         prof_begin(7, st);
-        if (fail(launch_init(ws.ctl, kMaxLevels, as_leaf ? ws.leaves : ws.segs[0], 1, n,
-                             SEG_RAW | (as_leaf ? 0x100u : 0u), st),
+        if (fail(launch_init(ws.ctl, kMaxLevels, as_huge ? ws.huge : (as_leaf ? ws.leaves : ws.segs[0]), 1, n,
+                             SEG_RAW | (as_huge ? kInitHuge : (as_leaf ? kInitLeaf : 0u)), st),
                  "init"))
             return GQS_EDEVICE;
         prof_end(st);
@@ -290,6 +300,9 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     a.mat_cap = (uint32_t)ws.cap.mat;
     a.bkt_cap = (uint32_t)ws.cap.bkt;
     a.seed = 0x5EEDu;
+    a.leaf_cap = leaf_cap(dtype);
+    a.huge = ws.huge;
+    a.huge_cap = (uint32_t)ws.cap.huge;
     // Level-0 leaves: the whole input when it fits one local sort, or the
     // short arrays of a segmented sort (all still in caller representation).
     if (offsets || L == 0) {
@@ -301,6 +314,9 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         la.lv = a;
         la.lv.ctl = &ws.ctl[0];
         la.leaf_max = p.leaf_max;
+        la.leaf_cap = leaf_cap(dtype);
+        la.huge = ws.huge;
+        la.huge_count = &ws.ctl[0].nhuge;
         la.is_last = 1;
         la.sort_threshold = p.sort_threshold;
         la.bitonic_cutoff = p.bitonic_cutoff;
@@ -329,6 +345,9 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         la.list_count = nullptr;
         la.lv = a;
         la.leaf_max = p.leaf_max;
+        la.leaf_cap = leaf_cap(dtype);
+        la.huge = ws.huge;
+        la.huge_count = &ws.ctl[lvl].nhuge;
         la.is_last = a.is_last;
         la.sort_threshold = p.sort_threshold;
         la.bitonic_cutoff = p.bitonic_cutoff;
@@ -379,7 +398,7 @@ gqs_status gqs_default_params(int dtype, gqs_params* out) {
     if (!out) return GQS_EINVAL;
     gqs_params p{};
     p.mode = GQS_MODE_MULTIPIVOT;
-    p.leaf_max = (dtype == DT_F64) ? 4096 : 8192;
+    p.leaf_max = 2 * leaf_cap(dtype);
     p.leaf_target = (dtype == DT_F64) ? 2048 : 4096;
     p.oversample = 16;
     p.max_fanout = 256;
@@ -448,7 +467,7 @@ gqs_status gqs_set_params(gqs_ctx* c, const gqs_params* p) {
     std::lock_guard<std::mutex> lk(c->mu);
     const bool pow2 = p->max_fanout >= 2 && p->max_fanout <= 256 && (p->max_fanout & (p->max_fanout - 1)) == 0;
     if ((p->mode != GQS_MODE_MULTIPIVOT && p->mode != GQS_MODE_CLASSIC) || p->leaf_max < 16 ||
-        p->leaf_max > 8192 || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
+        p->leaf_max > 2 * leaf_cap(DT_U32) || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
         p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32 ||
         p->sort_threshold < 4 || p->sort_threshold > 4096 || p->bitonic_cutoff < 64 || p->bitonic_cutoff > 512)
         return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");

@@ -14,8 +14,12 @@
 namespace gqs {
 
 struct Caps {
-    uint64_t seg = 0, mat = 0, bkt = 0, part = 0, leaves = 0, fill = 0;
+    uint64_t seg = 0, mat = 0, bkt = 0, part = 0, leaves = 0, fill = 0, huge = 0;
 };
+
+// Largest leaf one 512-thread work-group sorts (keys); leaves up to twice
+// that go to the 1024-thread instantiation through the large-leaf list.
+inline uint32_t leaf_cap(int dtype) { return dtype == DT_F64 ? 4096u : 8192u; }
 
 // Device workspace, grown on demand and reused across sorts.
 struct Workspace {
@@ -30,6 +34,7 @@ struct Workspace {
     FillTask* fill = nullptr;
     LevelCtl* ctl = nullptr;
     SegRec* leaves = nullptr;
+    SegRec* huge = nullptr;  // large-leaf list (reused by every level)
     Caps cap;
     size_t key_bytes = 0;
     void release();

@@ -45,6 +45,9 @@ struct LevelArgs {
     uint32_t bkt_cap;        // capacity of bkt
     uint32_t fill_cap;       // capacity of fill
     uint32_t seed;           // sample-position hash seed
+    uint32_t leaf_cap;       // largest leaf of one 512-thread work-group (keys)
+    SegRec* huge;            // leaves above leaf_cap (and <= leaf_max), appended by k_plan
+    uint32_t huge_cap;       // capacity of huge
 };
 
 // Local-sort (leaf) arguments.  Either an explicit list of segments
@@ -59,6 +62,10 @@ struct LeafArgs {
     uint32_t is_last;
     uint32_t sort_threshold;   // sub-bucket size the local partition aims for
     uint32_t bitonic_cutoff;   // largest sub-bucket one wave sorts (<= 512)
+    uint32_t leaf_cap;         // largest leaf of one 512-thread work-group; larger
+                               // ones (<= leaf_max) come from the large-leaf list
+    const SegRec* huge;        // the large-leaf list of this pass, count in *huge_count
+    const uint32_t* huge_count;
 };
 
 struct Grid {
@@ -67,15 +74,19 @@ struct Grid {
     int scan;       // work-groups of the scan kernels
     int plan;       // work-groups of the plan kernel
     int leaf;       // work-groups of the local sort
+    int huge;       // work-groups of the large-leaf sort (one per CU)
 };
 
 // Launchers (all stream-ordered, no host synchronisation).
+// flags: SEG_* of the single range, plus kInitLeaf (the range is a leaf of
+// the leaf list) or kInitHuge (a leaf of the large-leaf list).
+constexpr uint32_t kInitLeaf = 0x100u, kInitHuge = 0x200u;
 hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n,
                        uint32_t flags, hipStream_t st);
 hipError_t launch_leaf(int dtype, const LeafArgs& a, const Grid& g, hipStream_t st);
 hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
                                 LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max,
-                                hipStream_t st);
+                                uint32_t leaf_cap, SegRec* huge, hipStream_t st);
 
 // Merge of two sorted runs a[0, na) and b[0, nb) into out[0, na + nb) (out
 // overlaps neither run).  `split` holds merge_split_entries(dtype, na + nb)

@@ -458,10 +458,10 @@ __global__ void k_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_c
     for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) w[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0 && seg0_count) {
-        seg0[0] = SegRec{0, (uint32_t)n, flags};
-        if (flags & 0x100u) ctl[0].nleaf_list = 1;  // the single range is a leaf
+        seg0[0] = SegRec{0, (uint32_t)n, flags & 0xFFu};
+        if (flags & kInitHuge) ctl[0].nhuge = 1;  // the single range is a large leaf
+        else if (flags & kInitLeaf) ctl[0].nleaf_list = 1;  // ... a leaf
         else ctl[0].nseg = 1;
-        seg0[0].flags &= 0xFFu;
     }
 }
 
@@ -469,7 +469,8 @@ __global__ void k_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_c
 // nseg_dev, when given, holds the number of arrays (at most nseg_in, the
 // capacity the workspace was sized for) and is read when the kernel runs.
 __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
-                       LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max) {
+                       LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max, uint32_t leaf_cap,
+                       SegRec* huge) {
     const uint32_t nseg = nseg_dev ? min(*nseg_dev, nseg_in) : nseg_in;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += gridDim.x * blockDim.x) {
         const uint64_t lo = offsets[i], hi = offsets[i + 1];
@@ -477,6 +478,7 @@ __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t
         if (len < 2) continue;
         SegRec r{lo, (uint32_t)len, SEG_RAW};
         if (len > leaf_max) segs[atomicAdd(&ctl0->nseg, 1u)] = r;
+        else if (len > leaf_cap) huge[atomicAdd(&ctl0->nhuge, 1u)] = r;
         else leaves[atomicAdd(nleaves, 1u)] = r;
     }
 }
@@ -958,6 +960,14 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
                     a.next_segs[slot] = SegRec{seg.start + rel, size, (seg.flags & SEG_SRC_DN) ^ SEG_SRC_DN};
                 } else {
                     atomicOr(&a.ctl_next->overflow, 2u);
+                }
+            } else if (!eq && size > a.leaf_cap && size <= a.leaf_max) {
+                // a leaf too large for one 512-thread work-group: the large-leaf list
+                const uint32_t slot = atomicAdd(&a.ctl->nhuge, 1u);
+                if (slot < a.huge_cap) {
+                    a.huge[slot] = SegRec{seg.start + rel, size, (seg.flags & SEG_SRC_DN) ^ SEG_SRC_DN};
+                } else {
+                    atomicOr(&a.ctl->overflow, 8u);
                 }
             }
         }
@@ -1481,7 +1491,7 @@ __device__ __forceinline__ void wave_sort_any(uint32_t n, LD ld, ST st) {
 struct LeafDesc {
     uint64_t start;
     uint32_t n;
-    uint32_t kind;  // 0 none, 1 wave, 2 work-group, 3 unbounded
+    uint32_t kind;  // 0 none, 1 wave, 2 work-group, 3 unbounded, 4 large (1024-thread work-group)
     uint32_t from_dn;
     uint32_t raw;
 };
@@ -1507,7 +1517,8 @@ __device__ __forceinline__ LeafDesc leaf_decode(const LeafArgs& a, bool list_mod
         ds.n = r.z;
         ds.from_dn = r.w;
     }
-    if (ds.n >= 1) ds.kind = (ds.n <= a.bitonic_cutoff) ? 1u : (ds.n <= a.leaf_max ? 2u : 3u);
+    if (ds.n >= 1)
+        ds.kind = (ds.n <= a.bitonic_cutoff) ? 1u : (ds.n <= a.leaf_cap ? 2u : (ds.n <= a.leaf_max ? 4u : 3u));
     return ds;
 }
 __device__ __forceinline__ LeafDesc uniform_desc(const LeafDesc& d) {
@@ -1582,13 +1593,20 @@ __device__ __forceinline__ void copy_to_lds_async(K* dst, const K* src, uint32_t
 // its bucket in registers with an odd-even merge network; buckets above the
 // register capacity (skewed or duplicate-heavy leaves) go to one wave each,
 // or to the work-group in LDS.
-template <int DT>
-__global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
+//
+// kHuge: the same kernel with 1024 threads, twice the capacity and one local
+// bucket per thread, for the few leaves between the 512-thread capacity and
+// leaf_max (kind 4, read from the large-leaf list), so that buckets slightly
+// above that capacity need no further partition level.
+template <int DT, bool kHuge>
+__global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
     using K = typename KeyOf<DT>::K;
-    constexpr int NT = kLeafBigWG;
+    constexpr int NT = kHuge ? 2 * kLeafBigWG : kLeafBigWG;
     constexpr int NW = NT / 64;
-    constexpr uint32_t CAP = (sizeof(K) == 4) ? 8192u : 4096u;  // largest leaf
-    constexpr int KPT = (int)CAP / NT;                            // keys per thread
+    constexpr uint32_t CAP = ((sizeof(K) == 4) ? 8192u : 4096u) * (kHuge ? 2u : 1u);  // largest leaf
+    constexpr int KPT = (int)CAP / NT;                                                // keys per thread
+    constexpr int LL = NT;  // local buckets (one per thread)
+    constexpr uint32_t kMine = kHuge ? 4u : 2u;  // the leaf kind this kernel sorts
     // The small tables lead their LDS block so that their addresses stay
     // below 64 KiB, within reach of the immediate offset of LDS instructions.
     // P is an LDS variable of its own: the compiler then knows that plain LDS
@@ -1597,13 +1615,13 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
     // them (LDS atomics it waits for regardless; those here are inline asm).
     struct Lds {
         LeafDesc sd[3];
-        uint32_t hist[kLeafLocal + 1];
-        uint32_t bst[kLeafLocal + 1];
+        uint32_t hist[LL + 1];
+        uint32_t bst[LL + 1];
         uint32_t s_nbig;
         uint32_t junk[64];  // targets of the cursor adds of lanes without a key, one per lane
         uint32_t scanw[NW];
         K wmin[NW], wmax[NW];
-        uint16_t bigs16[kLeafLocal];
+        uint16_t bigs16[LL];
         alignas(16) K A[CAP];  // the leaf being sorted, written out next iteration
     };
     __shared__ Lds S;
@@ -1623,7 +1641,9 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
     const uint32_t G = gridDim.x;
     K* dout = (K*)a.d;
     auto big_only = [](LeafDesc ds) {
-        if (ds.kind < 2) ds.kind = 0;  // the wave kernel's leaves
+        // kinds sorted elsewhere: 1 (the wave kernel), 4 or 2 (the other
+        // instantiation); the unbounded fallback (3) belongs to the 512-thread one
+        if (ds.kind != kMine && (kHuge || ds.kind != 3u)) ds.kind = 0;
         return ds;
     };
     // Thread 0: record of g_{i+1}, in flight.  Loaded from a clamped index
@@ -1644,7 +1664,7 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
     lds_sync();
     {
         const LeafDesc c = uniform_desc(sd[0]);
-        if (c.kind == 2) copy_to_lds_async<K, NT>(P, (c.from_dn ? (const K*)a.dn : (const K*)a.d) + c.start, c.n, tid);
+        if (c.kind == kMine) copy_to_lds_async<K, NT>(P, (c.from_dn ? (const K*)a.dn : (const K*)a.d) + c.start, c.n, tid);
     }
     LeafDesc pend{0, 0, 0, 0, 0};  // sorted in A, not yet written out
     uint32_t g = blockIdx.x;
@@ -1677,7 +1697,7 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
         const LeafDesc nxt = uniform_desc(sd[(i + 1) % 3]);
         // 1. keys of g_i from P into registers, with their minimum and maximum
         K kv[KPT];
-        if (cur.kind == 2) {
+        if (cur.kind == kMine) {
             K mn = key_max<K>(), mx = K(0);
             // every slot is read (P holds CAP keys), so the reads issue
             // together; slots past the leaf are masked afterwards
@@ -1700,10 +1720,10 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
             }
         }
         write_out_pending();
-        if (tid < kLeafLocal) hist[tid] = 0;
+        if (tid < LL) hist[tid] = 0;
         if (tid == 0) s_nbig = 0;
         lds_sync();  // P consumed and A written out by every wave
-        if (nxt.kind == 2)
+        if (nxt.kind == kMine)
             copy_to_lds_async<K, NT>(P, (nxt.from_dn ? (const K*)a.dn : (const K*)a.d) + nxt.start, nxt.n, tid);
         if (tid == 0) fetch(g + 2 * G);
         if (cur.kind == 3) {
@@ -1719,7 +1739,7 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
             if (tid == 0) atomicAdd(&a.lv.ctl->oversize, 1u);
             continue;
         }
-        if (cur.kind != 2) continue;
+        if (cur.kind != kMine) continue;
         const uint32_t n = cur.n;
         K mn = wmin[0], mx = wmax[0];
 #pragma unroll
@@ -1737,10 +1757,10 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
         //    sort_threshold keys each.  Every key takes its rank inside its
         //    bucket from a returning LDS add (order inside a bucket is free:
         //    the bucket is sorted next) and keeps it with the bucket index as
-        //    b << 13 | rank (b < 512, rank < CAP <= 8192), so the placement
-        //    needs neither a second atomic nor the bucket again
+        //    b << kRankBits | rank (b < LL, rank < CAP = 2^kRankBits), so the
+        //    placement needs neither a second atomic nor the bucket again
         uint32_t nb = 2;
-        while (nb < (uint32_t)kLeafLocal && nb * a.sort_threshold < n) nb <<= 1;
+        while (nb < (uint32_t)LL && nb * a.sort_threshold < n) nb <<= 1;
         const double scale = (double)nb / ((double)(mx - mn) + 1.0);
         auto local_bucket = [&](K v) -> uint32_t {
             uint32_t b;
@@ -1749,7 +1769,9 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
             return min(b, nb - 1);
         };
         static_assert(KPT % 4 == 0, "rank adds go in fours");
-        static_assert(CAP <= 8192u && kLeafLocal <= 512, "bucket and rank share one word");
+        constexpr int kRankBits = Log2<(int)CAP>::v;
+        static_assert((1u << kRankBits) == CAP && ((uint64_t)LL << kRankBits) <= (1ull << 32),
+                      "bucket and rank share one word");
         uint32_t br[KPT];
 #pragma unroll
         for (int j = 0; j < KPT; ++j) br[j] = 0;
@@ -1764,7 +1786,7 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
                 }
                 lds_add_rtn4(ad[0], ad[1], ad[2], ad[3], 1u, rk[0], rk[1], rk[2], rk[3]);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) br[j + u] = (br[j + u] << 13) | rk[u];
+                for (int u = 0; u < 4; ++u) br[j + u] = (br[j + u] << kRankBits) | rk[u];
             }
         }
         lds_sync();
@@ -1777,7 +1799,7 @@ __global__ __launch_bounds__(kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
         // 4. place keys by bucket
 #pragma unroll
         for (int j = 0; j < KPT; ++j)
-            if (j * NT + tid < n) A[bst[br[j] >> 13] + (br[j] & 0x1FFFu)] = kv[j];
+            if (j * NT + tid < n) A[bst[br[j] >> kRankBits] + (br[j] & (CAP - 1u))] = kv[j];
         lds_sync();
         // 5. each thread sorts its bucket in registers with an odd-even merge
         //    network sized to the largest bucket of its wave
@@ -1833,10 +1855,10 @@ hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_c
 
 hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
                                 LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max,
-                                hipStream_t st) {
+                                uint32_t leaf_cap, SegRec* huge, hipStream_t st) {
     const uint32_t blocks = std::max(1u, std::min(1024u, (nseg_in + 255) / 256));
     hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(256), 0, st, offsets, nseg_in, nseg_dev, segs, ctl0, leaves,
-                       nleaves, leaf_max);
+                       nleaves, leaf_max, leaf_cap, huge);
     return hipGetLastError();
 }
 
@@ -1872,7 +1894,13 @@ template <int DT>
 static hipError_t leaf_t(const LeafArgs& a, const Grid& g, hipStream_t st) {
     hipLaunchKernelGGL((k_leaf_small<DT>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
     if (a.leaf_max > a.bitonic_cutoff || a.is_last)
-        hipLaunchKernelGGL((k_leaf_big<DT>), dim3(g.leaf), dim3(kLeafBigWG), 0, st, a);
+        hipLaunchKernelGGL((k_leaf_big<DT, false>), dim3(g.leaf), dim3(kLeafBigWG), 0, st, a);
+    if (a.leaf_max > a.leaf_cap) {  // leaves above the 512-thread capacity: the large-leaf list
+        LeafArgs h = a;
+        h.list = a.huge;
+        h.list_count = a.huge_count;
+        hipLaunchKernelGGL((k_leaf_big<DT, true>), dim3(g.huge), dim3(2 * kLeafBigWG), 0, st, h);
+    }
     return hipGetLastError();
 }
 

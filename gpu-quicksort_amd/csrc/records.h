@@ -68,6 +68,7 @@ struct LevelCtl {
     uint32_t nfill;          // equality-bucket fill tasks of this level
     uint32_t keys_lo;        // keys in this level's segments (low / high word)
     uint32_t keys_hi;
+    uint32_t nhuge;          // leaves of this level in the large-leaf list
 };
 
 // One chunk of an equality bucket to fill in d with its key value (the

@@ -57,7 +57,9 @@ typedef struct gqs_ctx gqs_ctx;
  * run-time parameters chosen by the autotuner). */
 typedef struct gqs_params {
     int32_t mode;          /* gqs_mode */
-    uint32_t leaf_max;     /* largest bucket sorted in LDS (QUICKSORT_BLOCK_SIZE role); 16..8192 (u32/f32); f64 uses at most 4096 */
+    uint32_t leaf_max;     /* largest bucket sorted in LDS (QUICKSORT_BLOCK_SIZE role); 16..16384 (u32/f32); f64 uses at
+                              most 8192.  Leaves above 8192 (f64: 4096) keys are sorted by a 1024-thread
+                              work-group, smaller ones by a 512-thread one */
     uint32_t leaf_target;  /* multipivot: bucket size the fan-out aims for */
     uint32_t oversample;   /* multipivot: sample keys per bucket */
     uint32_t max_fanout;   /* multipivot: largest fan-out, power of two in [2, 256] */

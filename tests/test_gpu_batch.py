@@ -26,7 +26,8 @@ def check_batch(got, keys, offs, case):
         assert bad == 0, f"{case}: array {i} [{lo},{hi}) {bad} keys differ, first at {first}"
 
 
-EDGE_LENS = [0, 1, 2, 3, 17, 64, 65, 511, 512, 513, 4095, 8192, 8193, 20000, 100000, 0, 5]
+EDGE_LENS = [0, 1, 2, 3, 17, 64, 65, 511, 512, 513, 4095, 4096, 4097, 8192, 8193, 12000, 16384, 16385, 20000,
+             100000, 0, 5]
 
 
 @pytest.mark.parametrize("dtype", [gqsort.U32, gqsort.F32, gqsort.F64])

@@ -110,3 +110,36 @@ def test_large_2p24_properties(gpu_ctx, gen):
     print(f"case 2^24 {gen}", flush=True)
     got = gpu_sort(gpu_ctx, a)
     check(got, a, f"2^24/{gen}")
+
+
+@pytest.mark.parametrize("dt,sizes", [(gqsort.U32, [8193, 12345, 16384]), (gqsort.F32, [8193, 16384]),
+                                      (gqsort.F64, [4097, 6001, 8192])])
+@pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
+def test_large_leaf_sizes(gpu_ctx, dt, sizes, gen):
+    """Arrays above one 512-thread work-group's capacity (8192 keys, f64 4096)
+    and up to leaf_max (twice that) are one leaf for the 1024-thread kernel."""
+    gpu_ctx.set_params(gqsort.default_params(dt))
+    try:
+        for n in sizes:
+            g = gen if dt == gqsort.U32 else ("normal" if gen in ("uniform", "bucket") else gen)
+            if dt != gqsort.U32 and g in ("sorted", "equal"):
+                a = np.sort(gqsort.generate(dt, "normal", 9, n)) if g == "sorted" else np.full(n, 1.5, gqsort.NP_DTYPES[dt])
+            else:
+                a = gqsort.generate(dt, g, 9, n)
+            print(f"case large leaf dt={dt} {gen} n={n}", flush=True)
+            check(gpu_sort(gpu_ctx, a), a, f"large leaf dt={dt} {gen} n={n}")
+            s = gpu_ctx.stats()
+            assert s.levels_launched == 0 and s.overflow == 0 and s.oversize_leaves == 0
+    finally:
+        gpu_ctx.set_params(gqsort.default_params())
+
+
+def test_large_leaves_from_a_level(gpu_ctx):
+    """Level buckets between the 512-thread capacity and leaf_max go to the
+    large-leaf list (leaf_target chosen so that many buckets land there)."""
+    a = gqsort.generate(gqsort.U32, "uniform", 12, 1 << 24)
+    got = gpu_sort(gpu_ctx, a, leaf_target=8192, max_fanout=256)
+    check(got, a, "large leaves from a level")
+    s = gpu_ctx.stats()
+    assert s.overflow == 0 and s.oversize_leaves == 0
+    gpu_ctx.set_params(gqsort.default_params())
