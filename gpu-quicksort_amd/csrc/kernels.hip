@@ -44,6 +44,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 #include "kernels.h"
@@ -813,30 +814,40 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
         // value masked afterwards): loads under a branch cannot be counted,
         // and the compiler would then wait for the next sub-tile's prefetch
         // before classifying the current one.
-        auto load = [&](uint32_t idx) {
-            const K v = load_key<DT>(base, min(idx, hi - 1u), raw);
-            return idx < hi ? v : K(0);
-        };
-        K keys[TS::kKeysPerThread];
+        // A full tile (every tile but a segment's last) needs no per-key
+        // bounds: only its final prefetch, past the tile, is clamped, and
+        // those values are never used.
+        auto count_tile = [&](auto full_tile) {
+            constexpr bool kFull = decltype(full_tile)::value;
+            auto load = [&](uint32_t idx) {
+                const K v = load_key<DT>(base, min(idx, hi - 1u), raw);
+                if constexpr (kFull) return v;
+                else return idx < hi ? v : K(0);
+            };
+            auto valid = [&](uint32_t idx) { return kFull || idx < hi; };
+            K keys[TS::kKeysPerThread];
 #pragma unroll
-        for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = load(lo + j * kPartWG + tid);
-        for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
-            // prefetch the next sub-tile while this one is classified
-            K nxt[TS::kKeysPerThread];
-            const uint32_t nsub = sub + TS::kSubKeys;
+            for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = load(lo + j * kPartWG + tid);
+            for (uint32_t sub = lo; sub < hi; sub += TS::kSubKeys) {
+                // prefetch the next sub-tile while this one is classified
+                K nxt[TS::kKeysPerThread];
+                const uint32_t nsub = sub + TS::kSubKeys;
 #pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) nxt[j] = load(nsub + j * kPartWG + tid);
-            uint32_t bk[TS::kKeysPerThread];
-            classify_lut<K, TS::kKeysPerThread>(keys, bk, L, p);
-            {
-                const bool try_peel = clustered(bk[0], sub + tid < hi);
+                for (int j = 0; j < TS::kKeysPerThread; ++j) nxt[j] = load(nsub + j * kPartWG + tid);
+                uint32_t bk[TS::kKeysPerThread];
+                classify_lut<K, TS::kKeysPerThread>(keys, bk, L, p);
+                {
+                    const bool try_peel = clustered(bk[0], valid(sub + tid));
 #pragma unroll
-                for (int j = 0; j < TS::kKeysPerThread; ++j)
-                    hist_add2(L.hist, bk[j], sub + j * kPartWG + tid < hi, try_peel);
+                    for (int j = 0; j < TS::kKeysPerThread; ++j)
+                        hist_add2(L.hist, bk[j], valid(sub + j * kPartWG + tid), try_peel);
+                }
+#pragma unroll
+                for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = nxt[j];
             }
-#pragma unroll
-            for (int j = 0; j < TS::kKeysPerThread; ++j) keys[j] = nxt[j];
-        }
+        };
+        if (hi - lo == (uint32_t)TS::kTileKeys) count_tile(std::true_type{});
+        else count_tile(std::false_type{});
         __syncthreads();
         for (uint32_t b = tid; b < p.nbuckets; b += kPartWG) a.mat[p.mat_base + b * p.ntiles + tl] = L.hist[b];
         __syncthreads();
