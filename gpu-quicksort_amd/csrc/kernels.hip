@@ -1772,11 +1772,19 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
         //    placement needs neither a second atomic nor the bucket again
         uint32_t nb = 2;
         while (nb < (uint32_t)LL && nb * a.sort_threshold < n) nb <<= 1;
-        const double scale = (double)nb / ((double)(mx - mn) + 1.0);
+        // b = (key - min) * nb / (range + 1) in float.  64-bit offsets are
+        // first shifted down to 32 significant bits (a monotone map; u64 ->
+        // double conversion has no instruction), so buckets stay monotone in
+        // the key and the clamp keeps them below nb.
+        const K rng = mx - mn;
+        uint32_t sh = 0;
+        if constexpr (sizeof(K) == 8) {
+            const uint32_t rhi = (uint32_t)(rng >> 32);
+            sh = rhi ? 32u - (uint32_t)__builtin_clz(rhi) : 0u;
+        }
+        const float scale = (float)nb / ((float)(uint32_t)(rng >> sh) + 1.0f);
         auto local_bucket = [&](K v) -> uint32_t {
-            uint32_t b;
-            if constexpr (sizeof(K) == 4) b = (uint32_t)((float)(v - mn) * (float)scale);
-            else b = (uint32_t)((double)(v - mn) * scale);
+            const uint32_t b = (uint32_t)((float)(uint32_t)((v - mn) >> sh) * scale);
             return min(b, nb - 1);
         };
         static_assert(KPT % 4 == 0, "rank adds go in fours");
