@@ -135,3 +135,26 @@ def test_graph_capture_in_fresh_tuned_context(dtype):
                     b.free()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("dtype", [gqsort.U32, gqsort.F32, gqsort.F64])
+def test_segmented_random_lengths(gpu_ctx, dtype):
+    """1500 arrays with random lengths across every size threshold (wave
+    leaves, 512-thread leaves, large leaves, partitioned arrays), in random
+    order, each checked against std::sort."""
+    rng = np.random.default_rng(77 + dtype)
+    lens = np.floor(2.0 ** rng.uniform(0, np.log2(40000), 1500)).astype(np.uint64)
+    lens[rng.choice(lens.size, 40, replace=False)] = rng.choice(
+        [0, 1, 2, 511, 512, 513, 4095, 4096, 4097, 8191, 8192, 8193, 16383, 16384, 16385], 40)
+    keys, offs = make_batch(dtype, lens, 5)
+    print(f"case segmented random lengths dtype={dtype} total={keys.size}", flush=True)
+    d = gqsort.DeviceBuffer.from_array(gpu_ctx, keys)
+    dn = gqsort.DeviceBuffer(gpu_ctx, max(keys.nbytes, 8))
+    od = gqsort.DeviceBuffer.from_array(gpu_ctx, offs)
+    stream = gqsort.Stream(gpu_ctx)
+    gpu_ctx.sort_segmented(dtype, d, dn, od, lens.size, keys.size, stream)
+    stream.sync()
+    check_batch(d.download(keys.dtype, keys.size, stream), keys, offs, f"random lengths dtype={dtype}")
+    s = gpu_ctx.stats()
+    assert s.overflow == 0 and s.oversize_leaves == 0
+    stream.close()

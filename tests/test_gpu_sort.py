@@ -143,3 +143,24 @@ def test_large_leaves_from_a_level(gpu_ctx):
     s = gpu_ctx.stats()
     assert s.overflow == 0 and s.oversize_leaves == 0
     gpu_ctx.set_params(gqsort.default_params())
+
+
+@pytest.mark.parametrize("dt", [gqsort.U32, gqsort.F32, gqsort.F64])
+@pytest.mark.parametrize("distinct", [3, 17, 1000])
+def test_duplicate_heavy(gpu_ctx, dt, distinct):
+    """Few distinct values with skewed (Zipf-like) frequencies: repeated
+    splitters give equality buckets that k_fill writes, and waves whose keys
+    share a bucket take the aggregated histogram path."""
+    n = 3 << 20
+    rng = np.random.default_rng(distinct)
+    w = 1.0 / np.arange(1, distinct + 1) ** 1.2
+    idx = rng.choice(distinct, size=n, p=w / w.sum())
+    if dt == gqsort.U32:
+        vals = rng.choice(np.iinfo(np.uint32).max, size=distinct, replace=False).astype(np.uint32)
+    else:
+        vals = rng.standard_normal(distinct).astype(gqsort.NP_DTYPES[dt]) * 1000
+    a = vals[idx]
+    print(f"case duplicates dt={dt} distinct={distinct}", flush=True)
+    check(gpu_sort(gpu_ctx, a), a, f"duplicates dt={dt} distinct={distinct}")
+    s = gpu_ctx.stats()
+    assert s.overflow == 0 and s.oversize_leaves == 0
