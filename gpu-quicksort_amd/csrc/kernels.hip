@@ -352,6 +352,12 @@ __device__ void lds_bitonic(K* buf, uint32_t n, uint32_t tid, uint32_t nthreads)
     }
 }
 
+// Work-group of the splitter kernel: four sample keys per thread (4096-key
+// samples), so the gather, the sample sort and the lookup-table build each
+// take a few steps per thread.
+constexpr int kSplWG = 1024;
+static_assert(kSampleMax % kSplWG == 0, "whole sample slots per thread");
+
 template <class K>
 __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
 template <class K>
@@ -362,7 +368,7 @@ __device__ __forceinline__ void wave_bitonic(K (&v)[E]);  // below, with the loc
 template <class K, int E, int NB, int J>
 __device__ __forceinline__ void wb_half_cleaners(K (&v)[NB][E], uint32_t lane);
 
-// Sorts buf[0, NW*64*E) with the kPartWG threads (NW waves).  Wave w holds
+// Sorts buf[0, NW*64*E) with the kSplWG threads (NW waves).  Wave w holds
 // block w (B = 64*E keys, element i = lane*E + r in v[r]) in its registers
 // for the whole sort: it sorts the block there, and in each bitonic merge of
 // blocks only the compare distances that reach another block (the mirror
@@ -371,7 +377,7 @@ __device__ __forceinline__ void wb_half_cleaners(K (&v)[NB][E], uint32_t lane);
 // registers.  A 4096-key sample thus takes three LDS exchanges.
 template <class K, int E>
 __device__ __forceinline__ void sample_sort_blocks(K* buf, uint32_t tid) {
-    constexpr uint32_t NW = kPartWG / 64;
+    constexpr uint32_t NW = kSplWG / 64;
     constexpr uint32_t B = 64u * E;
     const uint32_t lane = tid & 63u, w = tid >> 6;
     K v[E];
@@ -414,17 +420,17 @@ __device__ __forceinline__ void sample_sort_blocks(K* buf, uint32_t tid) {
     exchange();
 }
 
-// Sorts the splitter sample buf[0, S) (S a power of two) with kPartWG threads;
+// Sorts the splitter sample buf[0, S) (S a power of two) with kSplWG threads;
 // buf is sorted and visible to the whole work-group on return.
 template <class K>
 __device__ void sample_sort(K* buf, uint32_t S, uint32_t tid) {
-    switch (S / kPartWG) {
+    switch (S / kSplWG) {
         case 1: sample_sort_blocks<K, 1>(buf, tid); break;
         case 2: sample_sort_blocks<K, 2>(buf, tid); break;
         case 4: sample_sort_blocks<K, 4>(buf, tid); break;
         case 8: sample_sort_blocks<K, 8>(buf, tid); break;
         case 16: sample_sort_blocks<K, 16>(buf, tid); break;
-        default: lds_bitonic<K>(buf, S, tid, kPartWG); break;
+        default: lds_bitonic<K>(buf, S, tid, kSplWG); break;
     }
 }
 
@@ -486,11 +492,11 @@ __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t
 
 // ------------------------------------------------------------ k_splitters
 template <int DT>
-__global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
+__global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
     __shared__ K samp[kSampleMax];
     __shared__ K cand[kSplStride];
-    __shared__ uint32_t wsum[kPartWG / 64];
+    __shared__ uint32_t wsum[kSplWG / 64];
     __shared__ uint16_t lutw[kLutCells + 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t nseg = a.ctl->nseg;
@@ -525,11 +531,11 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             while (S > n) S >>= 1;
             while (k > 2 && k > S / 2) k >>= 1;
             // stratified random sample: all of a thread's loads issue together
-            constexpr uint32_t kPer = kSampleMax / kPartWG;
+            constexpr uint32_t kPer = kSampleMax / kSplWG;
             K sv[kPer];
 #pragma unroll
             for (uint32_t e = 0; e < kPer; ++e) {
-                const uint32_t j = tid + e * kPartWG;
+                const uint32_t j = tid + e * kSplWG;
                 sv[e] = K(0);
                 if (j < S) {
                     const uint64_t r = hash32(a.seed, (uint32_t)seg.start ^ (uint32_t)(seg.start >> 32), j) % n;
@@ -539,11 +545,11 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             }
 #pragma unroll
             for (uint32_t e = 0; e < kPer; ++e)
-                if (tid + e * kPartWG < S) samp[tid + e * kPartWG] = sv[e];
+                if (tid + e * kSplWG < S) samp[tid + e * kSplWG] = sv[e];
             __syncthreads();
             sample_sort<K>(samp, S, tid);
             ncand = k - 1;
-            for (uint32_t i = tid; i < ncand; i += kPartWG) cand[i] = samp[((uint64_t)(i + 1) * S) / k];
+            for (uint32_t i = tid; i < ncand; i += kSplWG) cand[i] = samp[((uint64_t)(i + 1) * S) / k];
         }
         __syncthreads();
         // Unique splitters u[] with an equality flag for values repeated among
@@ -556,10 +562,10 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             eq = (first && tid + 1 < ncand && cand[tid + 1] == c) ? 1u : 0u;
         }
         uint32_t m;
-        const uint32_t uidx = block_excl_scan<kPartWG>(first, wsum, &m);
+        const uint32_t uidx = block_excl_scan<kSplWG>(first, wsum, &m);
         uint32_t neq;
         // eq flag moves with its unique index: scan eq over unique order.
-        const uint32_t eqbefore = block_excl_scan<kPartWG>(eq, wsum, &neq);
+        const uint32_t eqbefore = block_excl_scan<kSplWG>(eq, wsum, &neq);
         // Table entry i (0 <= i <= 255): splitter u[i] (or max padding) and
         // meta = base[i] | eq[i] << 8 with base[i] = i + #eq among u[0..i).
         // Written by the thread that owns unique index i.
@@ -568,7 +574,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
             meta[uidx] = (uint16_t)((uidx + eqbefore) | (eq << 8));
         }
         const uint32_t nb = m + 1 + neq;
-        for (uint32_t i = m + tid; i < (uint32_t)kSplStride; i += kPartWG) {
+        for (uint32_t i = m + tid; i < (uint32_t)kSplStride; i += kSplWG) {
             spl[i] = key_max<K>();
             meta[i] = (uint16_t)((i == m) ? (m + neq) : (nb - 1));
         }
@@ -580,7 +586,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
         uint32_t bitlen = 0;
         while (bitlen < 8 * sizeof(K) && (range >> bitlen) != 0) ++bitlen;
         const uint32_t shift = bitlen > (uint32_t)kLutBits ? bitlen - kLutBits : 0u;
-        for (uint32_t cidx = tid; cidx < (uint32_t)kLutCells; cidx += kPartWG) {
+        for (uint32_t cidx = tid; cidx < (uint32_t)kLutCells; cidx += kSplWG) {
             const K off = (K)cidx << shift;
             uint32_t cnt = m;
             if (off <= range) {
@@ -598,7 +604,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
         if (tid == 0) lutw[kLutCells] = (uint16_t)m;
         __syncthreads();
         uint32_t wmax = 1;
-        for (uint32_t cidx = tid; cidx < (uint32_t)kLutCells; cidx += kPartWG) {
+        for (uint32_t cidx = tid; cidx < (uint32_t)kLutCells; cidx += kSplWG) {
             wmax = max(wmax, (uint32_t)lutw[cidx + 1] - lutw[cidx] + 1u);
             a.lut[(size_t)s * kLutCells + cidx] = (uint8_t)lutw[cidx];
         }
@@ -610,7 +616,7 @@ __global__ __launch_bounds__(kPartWG) void k_splitters(LevelArgs a) {
         __syncthreads();
         if (tid == 0) {
             uint32_t w = 1;
-            for (int i = 0; i < kPartWG / 64; ++i) w = max(w, wsum[i]);
+            for (int i = 0; i < kSplWG / 64; ++i) w = max(w, wsum[i]);
             uint32_t ld = 0;
             while ((1u << ld) < w) ++ld;
             SegPlan p;
@@ -1884,7 +1890,7 @@ hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const
 template <int DT>
 static hipError_t level_step_t(int step, const LevelArgs& a, const Grid& g, hipStream_t st) {
     switch (step) {
-        case 0: hipLaunchKernelGGL(k_splitters<DT>, dim3(g.splitters), dim3(kPartWG), 0, st, a); break;
+        case 0: hipLaunchKernelGGL(k_splitters<DT>, dim3(g.splitters), dim3(kSplWG), 0, st, a); break;
         case 1: hipLaunchKernelGGL(k_tileplan, dim3(1), dim3(kPlanWG), 0, st, a); break;
         case 2: hipLaunchKernelGGL(k_count<DT>, dim3(g.part), dim3(kPartWG), 0, st, a); break;
         case 3: hipLaunchKernelGGL(k_scan_reduce, dim3(g.scan), dim3(256), 0, st, a); break;
