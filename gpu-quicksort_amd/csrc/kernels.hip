@@ -7,7 +7,8 @@
 //
 //   k_splitters  per segment: pivot(s).  MODE_CLASSIC takes the median of the
 //                first, middle and last key (CL12/QuicksortKernels.cl:218-225);
-//                MODE_MULTIPIVOT sorts a stratified sample in LDS and keeps up
+//                MODE_MULTIPIVOT sorts a stratified sample (wave register
+//                sorts, merges across waves through LDS) and keeps up
 //                to 255 quantile splitters, giving repeated values their own
 //                equality bucket (the generalisation of the reference's
 //                "== pivot" class, whose keys are filled in place, :214-216).
@@ -20,7 +21,9 @@
 //                output offset per bucket (replaces the per-parent atomics on
 //                sstart/send, :176-187; deterministic and XCD-safe).
 //   k_plan       per bucket: size, placement flag, child segments appended to
-//                the next level's queue (the host's news -> work/done split,
+//                the next level's queue, leaves above one 512-thread work-
+//                group's capacity to the large-leaf list (the host's news ->
+//                work/done split,
 //                :362-371, and CL20's relauncher_kernel, CL20/QuicksortKernels.cl:132-222).
 //   k_scatter    persistent over tiles: classify again, rank inside the tile
 //                with wave ballots + LDS atomics, stage the tile in LDS grouped
@@ -37,7 +40,14 @@
 //                k_leaf_big: one work-group per larger leaf, software-pipelined
 //                (next leaf copied global->LDS asynchronously while the current
 //                one is partitioned by interpolation into LDS and its local
-//                buckets sorted in registers by odd-even merge networks).
+//                buckets sorted in registers by odd-even merge networks);
+//                512 threads up to 8192 keys (f64: 4096), 1024 threads for
+//                the large-leaf list (up to twice that).
+//
+// Loads that a later wait would serialise are written unconditionally (index
+// clamped, value masked), and register networks take their sizes as template
+// parameters: both keep clang from waiting for every prefetch load on the
+// spot or indexing key registers at run time.
 //
 // Keys are sorted as unsigned integers: float/double bits go through an
 // order-preserving bijection on the first load and back on the final store.
@@ -1262,7 +1272,6 @@ constexpr int kWaveSortMax = 512;
 constexpr int kLeafSplit = 64;  // largest local fan-out
 
 constexpr int kLeafBigWG = 512;
-constexpr int kLeafLocal = 512;   // local buckets of a work-group leaf (one per thread)
 // largest bucket one thread sorts in registers (64 VGPRs of keys at most)
 template <class K>
 constexpr uint32_t lane_sort_max() { return sizeof(K) == 4 ? 32u : 16u; }
