@@ -36,6 +36,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <unistd.h>
+
 #include "gqsort.h"
 
 namespace {
@@ -199,6 +201,20 @@ int main(int argc, char** argv) {
     if (gqs_create(&ctx, 0) != GQS_OK) {
         fprintf(stderr, "no GPU: %s\n", ctx ? gqs_last_error(ctx) : "gqs_create failed");
         return 1;
+    }
+    // the tuned parameter table shipped next to this executable, when present
+    {
+        char exe[4096];
+        const ssize_t len = readlink("/proc/self/exe", exe, sizeof exe - 1);
+        if (len > 0) {
+            exe[len] = 0;
+            std::string table(exe);
+            table = table.substr(0, table.rfind('/') + 1) + "tuned_mi355x.json";
+            if (FILE* f = fopen(table.c_str(), "r")) {
+                fclose(f);
+                gqs_tune_load(ctx, table.c_str());
+            }
+        }
     }
     std::cout << "Time to create the GPU context: " << (seconds() - t0) * 1000 << " ms" << std::endl;
     if (vendor != "amd") printf("vendor '%s' requested; running on HIP device 0\n", vendor.c_str());
