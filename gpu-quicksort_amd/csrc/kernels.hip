@@ -72,6 +72,29 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
 // asynchronous global->LDS copies issued before it stay in flight, where the
 // fence of __syncthreads() would wait for the copies.
 __device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// A value that every lane holds (read from LDS or global memory at a uniform
+// address), moved to scalar registers: the compiler cannot tell that such a
+// load gives every lane the same value, and would keep it in vector
+// registers, live across the whole tile loop.
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni(uint64_t x) {
+    return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | uni((uint32_t)x);
+}
+__device__ __forceinline__ SegRec uni(const SegRec& r) { return SegRec{uni(r.start), uni(r.n), uni(r.flags)}; }
+__device__ __forceinline__ SegPlan uni(const SegPlan& p) {
+    SegPlan u;
+    u.tile_begin = uni(p.tile_begin);
+    u.ntiles = uni(p.ntiles);
+    u.mat_base = uni(p.mat_base);
+    u.nbuckets = uni(p.nbuckets);
+    u.bucket_base = uni(p.bucket_base);
+    u.has_eq = uni(p.has_eq);
+    u.lut_shift = uni(p.lut_shift);
+    u.lut_depth = uni(p.lut_depth);
+    u.lut_lo = uni(p.lut_lo);
+    return u;
+}
 // Wait for every global load, store and global->LDS copy this wave issued.
 __device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
@@ -811,15 +834,15 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
             L.seg = seg_i;
         }
         __syncthreads();
-        const uint32_t s = L.seg;
+        const uint32_t s = uni(L.seg);
         if (s != cur) {
             load_splitters<K>(a, s, L);
             cur = s;
         }
         L.hist[tid] = 0;
         __syncthreads();
-        const SegRec seg = a.segs[s];
-        const SegPlan p = a.plans[s];
+        const SegRec seg = uni(a.segs[s]);
+        const SegPlan p = uni(a.plans[s]);
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         const bool raw = seg.flags & SEG_RAW;
         const uint32_t tl = t - p.tile_begin;
@@ -1064,6 +1087,14 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     __shared__ uint32_t s_anydirect;
     __shared__ SegRec s_seg;  // the current segment's record and plan
     __shared__ SegPlan s_plan;
+    // Scanned count-matrix entries of kMatBlock consecutive tiles, per bucket
+    // (row stride kMatBlock + 1 against bank conflicts).  A tile's entries lie
+    // ntiles apart, one 128-byte line each; loaded tile by tile they were
+    // fetched again for every tile (the L2 turns over between two tiles of a
+    // work-group), half as many bytes again as the keys.  Loaded per block,
+    // each line serves kMatBlock tiles.
+    constexpr uint32_t kMatBlock = 32;
+    __shared__ uint32_t mblk[kSplStride * (kMatBlock + 1)];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = a.ctl->nseg;
@@ -1076,6 +1107,8 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     }
     if (tid < (uint32_t)kSplStride) L.hist[tid] = 0;  // afterwards zeroed by its owner right after each use
     uint32_t cur = 0xFFFFFFFFu;
+    uint32_t mb_seg = 0xFFFFFFFFu, mb_lo = 0, mb_hi = 0;  // mblk holds tiles [mb_lo, mb_hi) of mb_seg
+    uint32_t mb_base = 0;                                  // first entry of mb_seg's matrix
     K keys[KPT];
     bool have_keys = false;  // keys[] already hold this tile's first sub-tile
     for (uint32_t t = t0; t < t1; ++t) {
@@ -1088,7 +1121,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
             s_anydirect = 0;
         }
         __syncthreads();
-        const uint32_t s = L.seg;
+        const uint32_t s = uni(L.seg);
         if (s != cur) {
             load_splitters<K>(a, s, L);
             if (tid == 0) {
@@ -1098,14 +1131,38 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
             cur = s;
             __syncthreads();
         }
-        const SegRec seg = s_seg;
-        const SegPlan p = s_plan;
+        const SegRec seg = uni(s_seg);
+        const SegPlan p = uni(s_plan);
         const uint32_t tl = t - p.tile_begin;
+        if (s != mb_seg || tl >= mb_hi) {  // uniform; every thread has read the previous block
+            mb_seg = s;
+            mb_lo = uni(tl);
+            mb_hi = uni(min(min(tl + kMatBlock, p.ntiles), tl + (t1 - t)));
+            mb_base = uni(a.mat[p.mat_base]);
+            // every slot is loaded from a clamped, valid entry (rows past
+            // nbuckets and columns past the block are never read), so the
+            // loads issue together
+            const uint32_t* m = a.mat + p.mat_base;
+            constexpr int kPerThread = kSplStride * (int)kMatBlock / kScatterWG;
+            uint32_t mv[kPerThread];
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const uint32_t e = q * kScatterWG + tid;
+                const uint32_t b = min(e / kMatBlock, p.nbuckets - 1u), j = min(mb_lo + e % kMatBlock, mb_hi - 1u);
+                mv[q] = m[b * p.ntiles + j];
+            }
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const uint32_t e = q * kScatterWG + tid;
+                mblk[(e / kMatBlock) * (kMatBlock + 1) + e % kMatBlock] = mv[q];
+            }
+            __syncthreads();
+        }
         // Thread b owns bucket b: its output offset inside the segment, and
         // whether it is an equality bucket (loads consumed after the ranks).
         uint32_t my_cursor = 0, bsize = 0;
         if (tid < p.nbuckets) {
-            my_cursor = a.mat[p.mat_base + tid * p.ntiles + tl] - a.mat[p.mat_base];
+            my_cursor = mblk[tid * (kMatBlock + 1) + (tl - mb_lo)] - mb_base;
             bsize = a.bkt[p.bucket_base + tid].size;
         }
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
