@@ -121,7 +121,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 // call; callers that pass another barrier before reusing wsum may use it.
 template <int NT, bool kLdsOnly = false, bool kTrailing = true>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    // the wave index in a scalar register (uniform in the wave)
+    const uint32_t lane = lane_id(), w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t x = wave_incl_scan(v);
     if (lane == 63) wsum[w] = x;
     if constexpr (kLdsOnly) lds_sync();
@@ -1060,41 +1061,46 @@ __global__ __launch_bounds__(256) void k_fill(LevelArgs a) {
 // Persistent over tiles; every work-group takes a contiguous run of tiles, so
 // consecutive tiles mostly share a segment (splitters, segment record and
 // plan stay put) and thread 0 advances the segment index instead of searching
-// for it.  Per 4 Ki-key sub-tile: classify, rank inside the sub-tile, scan the
-// bucket counts, stage the keys grouped by bucket in LDS, store the runs.
-// The keys of the next sub-tile -- or of the next tile's first sub-tile when
-// it belongs to the same segment -- are loaded while the current one is
-// staged and stored.  Barriers per sub-tile: after the ranks (counts and
-// equality flags complete), inside the scan, after the scan (run starts), after
-// the staging; the next sub-tile's first barrier keeps its writes to shared
-// state behind every wave's stores of this one.
-constexpr int kScatterWG = 1024;  // threads of a scatter work-group: one staged sub-tile is a whole tile
+// for it.  One tile (kScatterWG * KPT keys) at a time: the tile's per-bucket
+// counts are the differences of adjacent entries of the scanned count matrix
+// (the count pass saw the same keys in the same tile), so their scan gives
+// the tile-local run starts before any key is ranked; the rank counters start
+// there and every returning LDS add hands its key the staging slot directly.
+// Then the keys are staged grouped by bucket and the runs stored.  The keys
+// of the next tile, when it belongs to the same segment and work-group, are
+// loaded while the current one is staged and stored.  Barriers per tile: the
+// segment/tile-start one, inside the scan, after the run starts (rank
+// counters set), after the staging; the next tile's first barrier keeps its
+// writes to shared state behind every wave's stores of this one.
+constexpr int kScatterWG = 1024;  // threads of a scatter work-group: one tile per pass
 template <int DT>
 __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
     using TS = TileShape<K>;
     constexpr int KPT = TS::kKeysPerThread;
-    constexpr uint32_t kSub = (uint32_t)kScatterWG * KPT;  // keys per staged sub-tile
-    static_assert(TS::kTileKeys % kSub == 0, "a tile is whole sub-tiles");
+    constexpr uint32_t kTile = (uint32_t)kScatterWG * KPT;  // keys per tile, staged at once
+    static_assert(TS::kTileKeys == (int)kTile, "the count pass's tile is one scatter pass");
+    static_assert(kTile <= 65536 && kSplStride <= 256, "bucket and slot share one word");
     constexpr bool kEarlyPrefetch = DT == DT_U32;  // keys need no transform
     __shared__ TileLds<K> L;
-    __shared__ uint32_t lstart[kSplStride];
     __shared__ uint32_t delta[kSplStride];  // output offset - staging offset, per bucket
     __shared__ uint8_t direct[kSplStride];
-    __shared__ K stage[kSub];
-    __shared__ uint8_t sbk[kSub];
+    __shared__ K stage[kTile];
+    __shared__ uint8_t sbk[kTile];
     __shared__ uint32_t wsum[kScatterWG / 64];
     __shared__ uint32_t s_anydirect;
     __shared__ SegRec s_seg;  // the current segment's record and plan
     __shared__ SegPlan s_plan;
-    // Scanned count-matrix entries of kMatBlock consecutive tiles, per bucket
-    // (row stride kMatBlock + 1 against bank conflicts).  A tile's entries lie
-    // ntiles apart, one 128-byte line each; loaded tile by tile they were
-    // fetched again for every tile (the L2 turns over between two tiles of a
-    // work-group), half as many bytes again as the keys.  Loaded per block,
-    // each line serves kMatBlock tiles.
-    constexpr uint32_t kMatBlock = 32;
-    __shared__ uint32_t mblk[kSplStride * (kMatBlock + 1)];
+    // Scanned count-matrix entries of kMatBlock consecutive tiles plus the one
+    // after them, per bucket (row stride kRow, odd against bank conflicts).  A
+    // tile's entries lie ntiles apart, one 128-byte line each; loaded tile by
+    // tile they were fetched again for every tile (the L2 turns over between
+    // two tiles of a work-group), half as many bytes again as the keys.
+    // Loaded per block, each line serves kMatBlock tiles.  Entry (b, ntiles)
+    // is entry (b + 1, 0), so column j + 1 - column j is the count of bucket
+    // b in tile j for every bucket but the last.
+    constexpr uint32_t kMatBlock = 32, kRow = kMatBlock + 1;
+    __shared__ uint32_t mblk[kSplStride * kRow];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = a.ctl->nseg;
@@ -1105,14 +1111,18 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
         seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
         seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
     }
-    if (tid < (uint32_t)kSplStride) L.hist[tid] = 0;  // afterwards zeroed by its owner right after each use
     uint32_t cur = 0xFFFFFFFFu;
     uint32_t mb_seg = 0xFFFFFFFFu, mb_lo = 0, mb_hi = 0;  // mblk holds tiles [mb_lo, mb_hi) of mb_seg
     uint32_t mb_base = 0;                                  // first entry of mb_seg's matrix
     K keys[KPT];
-    bool have_keys = false;  // keys[] already hold this tile's first sub-tile
+    bool have_keys = false;  // keys[] already hold this tile's keys
     for (uint32_t t = t0; t < t1; ++t) {
-        if (tid == 0) {
+        // tid, opaque to the compiler: values derived from it (LDS addresses)
+        // are recomputed per tile instead of being held, or spilled, across
+        // the whole loop
+        uint32_t ti;
+        asm volatile("" : "=v"(ti) : "0"(tid));
+        if (ti == 0) {
             while (t >= seg_next) {
                 ++seg_i;
                 seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
@@ -1124,7 +1134,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
         const uint32_t s = uni(L.seg);
         if (s != cur) {
             load_splitters<K>(a, s, L);
-            if (tid == 0) {
+            if (ti == 0) {
                 s_seg = a.segs[s];
                 s_plan = a.plans[s];
             }
@@ -1139,37 +1149,44 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
             mb_lo = uni(tl);
             mb_hi = uni(min(min(tl + kMatBlock, p.ntiles), tl + (t1 - t)));
             mb_base = uni(a.mat[p.mat_base]);
-            // every slot is loaded from a clamped, valid entry (rows past
-            // nbuckets and columns past the block are never read), so the
-            // loads issue together
+            // columns 0 .. mb_hi - mb_lo; every slot is loaded from a clamped,
+            // valid entry of the segment's matrix (rows past nbuckets, columns
+            // past the block and entry (nbuckets - 1, ntiles) are never read),
+            // so the loads issue together
+            // Four threads per bucket row, column (ti & 3) + 4q.
+            static_assert(kScatterWG == 4 * kSplStride, "four threads per matrix row");
             const uint32_t* m = a.mat + p.mat_base;
-            constexpr int kPerThread = kSplStride * (int)kMatBlock / kScatterWG;
-            uint32_t mv[kPerThread];
+            const uint32_t last = p.nbuckets * p.ntiles - 1u;
+            constexpr int kCols = (int)(kRow + 3) / 4;
+            const uint32_t r = ti >> 2, rb = min(r, p.nbuckets - 1u) * p.ntiles + mb_lo;
+            uint32_t mv[kCols];
 #pragma unroll
-            for (int q = 0; q < kPerThread; ++q) {
-                const uint32_t e = q * kScatterWG + tid;
-                const uint32_t b = min(e / kMatBlock, p.nbuckets - 1u), j = min(mb_lo + e % kMatBlock, mb_hi - 1u);
-                mv[q] = m[b * p.ntiles + j];
+            for (int q = 0; q < kCols; ++q) {
+                const uint32_t j = min((ti & 3u) + 4u * q, mb_hi - mb_lo);
+                mv[q] = m[min(rb + j, last)];
             }
 #pragma unroll
-            for (int q = 0; q < kPerThread; ++q) {
-                const uint32_t e = q * kScatterWG + tid;
-                mblk[(e / kMatBlock) * (kMatBlock + 1) + e % kMatBlock] = mv[q];
+            for (int q = 0; q < kCols; ++q) {
+                const uint32_t j = (ti & 3u) + 4u * q;
+                if (j < kRow) mblk[r * kRow + j] = mv[q];
             }
             __syncthreads();
         }
-        // Thread b owns bucket b: its output offset inside the segment, and
-        // whether it is an equality bucket (loads consumed after the ranks).
-        uint32_t my_cursor = 0, bsize = 0;
-        if (tid < p.nbuckets) {
-            my_cursor = mblk[tid * (kMatBlock + 1) + (tl - mb_lo)] - mb_base;
-            bsize = a.bkt[p.bucket_base + tid].size;
+        // Thread b owns bucket b: its output offset inside the segment, its
+        // key count in this tile, and whether it is an equality bucket.
+        uint32_t my_cursor = 0, mycnt = 0, bsize = 0;
+        if (ti < p.nbuckets) {
+            const uint32_t* row = &mblk[ti * kRow + (tl - mb_lo)];
+            my_cursor = row[0] - mb_base;
+            mycnt = (ti + 1 < p.nbuckets) ? row[1] - row[0] : 0u;  // the last run needs no count
+            bsize = a.bkt[p.bucket_base + ti].size;
         }
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         K* dst = ((seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn) + seg.start;
         const bool raw = seg.flags & SEG_RAW;
-        const uint32_t lo = tl * (uint32_t)TS::kTileKeys;
-        const uint32_t hi = min(lo + (uint32_t)TS::kTileKeys, seg.n);
+        const uint32_t lo = tl * kTile;
+        const uint32_t hi = min(lo + kTile, seg.n);
+        const uint32_t cnt = hi - lo;
         const K* base = src + seg.start;
         // Key loads are unconditional (index clamped into the range, value
         // masked afterwards) and the float key transform is applied only
@@ -1178,137 +1195,114 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
         if (!have_keys) {
             K raw_keys[KPT];
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) raw_keys[j] = base[min(lo + j * kScatterWG + tid, hi - 1u)];
+            for (int j = 0; j < KPT; ++j) raw_keys[j] = base[min(lo + j * kScatterWG + ti, hi - 1u)];
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
                 const K v = raw ? to_key<DT>(raw_keys[j]) : raw_keys[j];
-                keys[j] = (lo + j * kScatterWG + tid < hi) ? v : K(0);
+                keys[j] = (lo + j * kScatterWG + ti < hi) ? v : K(0);
             }
         }
-        have_keys = false;
-        bool anyd = false;
-        for (uint32_t sub = lo; sub < hi; sub += kSub) {
-            const bool first = sub == lo;
-            // bucket (< 256) and rank inside the sub-tile (< kSub) of each key,
-            // packed as bucket << 16 | rank to keep the registers free for the
-            // prefetched keys
-            static_assert(kSub <= 65536 && kSplStride <= 256, "bucket and rank share one word");
-            uint32_t br[KPT];
-            {
-                uint32_t bk[KPT];
-                uint32_t rk[KPT];
-                bool valid[KPT];
-                classify_lut<K, KPT>(keys, bk, L, p);
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) valid[j] = sub + j * kScatterWG + tid < hi;
-                hist_rank_n<KPT>(L.hist, bk, valid, rk);
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
-            }
-            // prefetch: the next sub-tile, or the next tile's first sub-tile
-            // when that tile is this work-group's and in the same segment.
-            // uint32 keys are loaded here, before the scan and the staging.
-            // Float keys are loaded once the current keys are staged (both
-            // sets live at once, with the transform pending, spill), as
-            // unconditional clamped loads whose transform and mask wait until
-            // the keys are used: a transform next to a load under a branch
-            // made the compiler wait for every prefetch load on the spot.
-            uint32_t nsub = sub + kSub, nhi = hi;
-            bool next_tile = false;
-            if (nsub >= hi) {
-                next_tile = t + 1 < t1 && tl + 1 < p.ntiles;
-                nsub = (tl + 1) * (uint32_t)TS::kTileKeys;
-                nhi = next_tile ? min(nsub + (uint32_t)TS::kTileKeys, seg.n) : 0u;
-            }
-            K nxt[KPT];
-            if constexpr (kEarlyPrefetch) {
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) {
-                    const uint32_t idx = nsub + j * kScatterWG + tid;
-                    nxt[j] = (idx < nhi) ? base[idx] : K(0);
-                }
-            }
-            if (first && tid < p.nbuckets) {
+        // run starts; the rank counters begin at them
+        {
+            uint32_t tot;
+            const uint32_t ls = block_excl_scan<kScatterWG, false, false>(mycnt, wsum, &tot);
+            if (ti < p.nbuckets) {
+                L.hist[ti] = ls;
+                delta[ti] = my_cursor - ls;
                 const bool dir = (bsize & kBucketDirect) != 0;
-                direct[tid] = dir ? 1 : 0;
+                direct[ti] = dir ? 1 : 0;
                 if (dir) s_anydirect = 1;
             }
-            __syncthreads();
-            if (first) anyd = s_anydirect != 0;
-            uint32_t cnt;
-            const uint32_t hv = (tid < p.nbuckets) ? L.hist[tid] : 0u;
-            if (tid < (uint32_t)kSplStride) L.hist[tid] = 0;
-            const uint32_t ls = block_excl_scan<kScatterWG, false, false>(hv, wsum, &cnt);
-            if (tid < (uint32_t)kSplStride) {
-                lstart[tid] = ls;
-                delta[tid] = my_cursor - ls;
+        }
+        __syncthreads();
+        const bool anyd = s_anydirect != 0;
+        // bucket (< 256) and staging slot (< kTile) of each key, packed as
+        // bucket << 16 | slot to keep the registers free for the prefetched keys
+        uint32_t br[KPT];
+        {
+            uint32_t bk[KPT];
+            uint32_t rk[KPT];
+            bool valid[KPT];
+            classify_lut<K, KPT>(keys, bk, L, p);
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) valid[j] = lo + j * kScatterWG + ti < hi;
+            hist_rank_n<KPT>(L.hist, bk, valid, rk);
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
+        }
+        // prefetch: the next tile when it is this work-group's and in the same
+        // segment.  uint32 keys are loaded here, before the staging.  Float
+        // keys are loaded once the current keys are staged (both sets live at
+        // once, with the transform pending, spill), as unconditional clamped
+        // loads whose transform and mask wait until the keys are used.
+        const bool next_tile = t + 1 < t1 && tl + 1 < p.ntiles;
+        const uint32_t nlo = lo + kTile;
+        const uint32_t nhi = next_tile ? min(nlo + kTile, seg.n) : 0u;
+        K nxt[KPT];
+        if constexpr (kEarlyPrefetch) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t idx = nlo + j * kScatterWG + ti;
+                nxt[j] = (idx < nhi) ? base[idx] : K(0);
             }
-            __syncthreads();
-            // stage by bucket: the lstart reads of half the keys issue together
+        }
+        // stage by bucket
 #pragma unroll
-            for (int j0 = 0; j0 < KPT; j0 += KPT / 2) {
-                uint32_t ps[KPT / 2];
-#pragma unroll
-                for (int j = 0; j < KPT / 2; ++j) ps[j] = lstart[br[j0 + j] >> 16];
-#pragma unroll
-                for (int j = 0; j < KPT / 2; ++j) {
-                    if (sub + (j0 + j) * kScatterWG + tid < hi) {
-                        const uint32_t pos = ps[j] + (br[j0 + j] & 0xFFFFu);
-                        stage[pos] = keys[j0 + j];
-                        sbk[pos] = (uint8_t)(br[j0 + j] >> 16);
-                    }
-                }
+        for (int j = 0; j < KPT; ++j) {
+            if (lo + j * kScatterWG + ti < hi) {
+                const uint32_t pos = br[j] & 0xFFFFu;
+                stage[pos] = keys[j];
+                sbk[pos] = (uint8_t)(br[j] >> 16);
             }
-            if constexpr (!kEarlyPrefetch) {
-                if (nsub < nhi) {  // uniform
+        }
+        if constexpr (!kEarlyPrefetch) {
+            if (next_tile) {  // uniform
 #pragma unroll
-                    for (int j = 0; j < KPT; ++j) nxt[j] = base[min(nsub + j * kScatterWG + tid, nhi - 1u)];
-                } else {
-#pragma unroll
-                    for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
-                }
-            }
-            __syncthreads();
-            // coalesced stores of the staged runs, W slots per thread at a
-            // time: bucket ids, then offsets and keys, each batch of LDS reads
-            // issued together (slots past cnt are read but not stored)
-            constexpr int W = 8;
-#pragma unroll
-            for (int u0 = 0; u0 < KPT; u0 += W) {
-                if ((uint32_t)u0 * kScatterWG >= cnt) break;
-                uint32_t sb[W], dl[W];
-                K sv[W];
-#pragma unroll
-                for (int u = 0; u < W; ++u) sb[u] = sbk[(u0 + u) * kScatterWG + tid];
-#pragma unroll
-                for (int u = 0; u < W; ++u) {
-                    dl[u] = delta[sb[u]];
-                    sv[u] = stage[(u0 + u) * kScatterWG + tid];
-                }
-                if (anyd) {
-#pragma unroll
-                    for (int u = 0; u < W; ++u)
-                        if (direct[sb[u]]) dl[u] = 0xFFFFFFFFu;  // equality buckets: k_fill
-                }
-#pragma unroll
-                for (int u = 0; u < W; ++u) {
-                    const uint32_t i = (u0 + u) * kScatterWG + tid;
-                    if (i < cnt && dl[u] != 0xFFFFFFFFu) dst[dl[u] + i] = sv[u];
-                }
-            }
-            my_cursor += hv;
-            if constexpr (kEarlyPrefetch) {
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+                for (int j = 0; j < KPT; ++j) nxt[j] = base[min(nlo + j * kScatterWG + ti, nhi - 1u)];
             } else {
 #pragma unroll
-                for (int j = 0; j < KPT; ++j) {
-                    const K v = raw ? to_key<DT>(nxt[j]) : nxt[j];
-                    keys[j] = (nsub + j * kScatterWG + tid < nhi) ? v : K(0);
-                }
+                for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
             }
-            have_keys = next_tile;
         }
+        __syncthreads();
+        // coalesced stores of the staged runs, W slots per thread at a time:
+        // bucket ids, then offsets and keys, each batch of LDS reads issued
+        // together (slots past cnt are read but not stored)
+        constexpr int W = 8;
+#pragma unroll
+        for (int u0 = 0; u0 < KPT; u0 += W) {
+            if ((uint32_t)u0 * kScatterWG >= cnt) break;
+            uint32_t sb[W], dl[W];
+            K sv[W];
+#pragma unroll
+            for (int u = 0; u < W; ++u) sb[u] = sbk[(u0 + u) * kScatterWG + ti];
+#pragma unroll
+            for (int u = 0; u < W; ++u) {
+                dl[u] = delta[sb[u]];
+                sv[u] = stage[(u0 + u) * kScatterWG + ti];
+            }
+            if (anyd) {
+#pragma unroll
+                for (int u = 0; u < W; ++u)
+                    if (direct[sb[u]]) dl[u] = 0xFFFFFFFFu;  // equality buckets: k_fill
+            }
+#pragma unroll
+            for (int u = 0; u < W; ++u) {
+                const uint32_t i = (u0 + u) * kScatterWG + ti;
+                if (i < cnt && dl[u] != 0xFFFFFFFFu) dst[dl[u] + i] = sv[u];
+            }
+        }
+        if constexpr (kEarlyPrefetch) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const K v = raw ? to_key<DT>(nxt[j]) : nxt[j];
+                keys[j] = (nlo + j * kScatterWG + ti < nhi) ? v : K(0);
+            }
+        }
+        have_keys = next_tile;
     }
 }
 
