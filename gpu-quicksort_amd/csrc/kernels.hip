@@ -25,9 +25,11 @@
 //                group's capacity to the large-leaf list (the host's news ->
 //                work/done split,
 //                :362-371, and CL20's relauncher_kernel, CL20/QuicksortKernels.cl:132-222).
-//   k_scatter    persistent over tiles: classify again, rank inside the tile
-//                with wave ballots + LDS atomics, stage the tile in LDS grouped
-//                by bucket and store coalesced runs (the scatter pass, :195-203).
+//   k_scatter    persistent over tiles: run starts from the tile's counts in
+//                the scanned count matrix, classify again, rank with wave
+//                ballots + returning LDS adds that give each key its staging
+//                slot, stage the tile in LDS grouped by bucket and store
+//                coalesced runs (the scatter pass, :195-203).
 //                Keys of equality buckets are not stored at all.
 //   k_fill       equality buckets: the splitter value written over the bucket's
 //                range of d, in caller representation, by many work-groups
