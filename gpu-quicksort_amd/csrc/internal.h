@@ -46,10 +46,11 @@ struct ProfRec {
 };
 
 // Steps of one partition level, in launch order, and their profiling kinds.
-constexpr int kLevelSteps = 8;
+constexpr int kLevelSteps = 7;
 inline int step_kind(int step) {
-    // splitters, tileplan, count, scan_reduce, scan_apply, plan, scatter, fill
-    static const int kinds[kLevelSteps] = {0, 1, 2, 3, 3, 4, 5, 8};
+    // splitters, tileplan, count, scan_reduce, scan_apply, plan, scatter
+    // (equality buckets are filled by the level's leaf pass)
+    static const int kinds[kLevelSteps] = {0, 1, 2, 3, 3, 4, 5};
     return kinds[step];
 }
 hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st);

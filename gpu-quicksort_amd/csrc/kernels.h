@@ -10,7 +10,7 @@ namespace gqs {
 // Per-bucket result of one partition level: where the bucket starts inside
 // its segment and how many keys it holds.  Bit 31 of `size` marks an
 // equality bucket: its keys all equal one splitter, the scatter does not store
-// them and k_fill writes the value into d after the scatter has finished.
+// them and the level's leaf pass writes the value into d after the scatter.
 struct BucketRec {
     uint64_t start;   // absolute index of the bucket's first key
     uint32_t size;    // keys (bit 31: equality bucket)

@@ -31,10 +31,11 @@
 //                slot, stage the tile in LDS grouped by bucket and store
 //                coalesced runs (the scatter pass, :195-203).
 //                Keys of equality buckets are not stored at all.
-//   k_fill       equality buckets: the splitter value written over the bucket's
-//                range of d, in caller representation, by many work-groups
-//                after the scatter (the pivot fill, :214-216, which the
-//                reference does serially in the last work-group).
+//   equality buckets: the splitter value written over the bucket's range of
+//                d, in caller representation, by the work-groups of the next
+//                k_leaf_small launch, after the level's scatter (the pivot
+//                fill, :214-216, which the reference does serially in the last
+//                work-group).
 //   k_leaf_*     persistent over buckets: local sort of every bucket of at most
 //                leaf_max keys (lqsort_kernel, :267-421), written to d in caller
 //                representation.  k_leaf_small: one wave per leaf of at most
@@ -1043,21 +1044,6 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
     }
 }
 
-// ------------------------------------------------------------ k_fill
-// Equality buckets: write the splitter value, in caller representation, over
-// each task's range of d.  Runs after the level's scatter, so no tile of the
-// level still reads these positions.
-template <int DT>
-__global__ __launch_bounds__(256) void k_fill(LevelArgs a) {
-    using K = typename KeyOf<DT>::K;
-    const uint32_t nt = min(a.ctl->nfill, a.fill_cap);
-    K* d = (K*)a.d;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        const FillTask f = a.fill[t];
-        const K v = from_key<DT>(((const K*)a.spl)[f.spl_index]);
-        for (uint32_t i = threadIdx.x; i < f.n; i += 256) d[f.start + i] = v;
-    }
-}
 
 // ------------------------------------------------------------ k_scatter
 // Persistent over tiles; every work-group takes a contiguous run of tiles, so
@@ -1286,7 +1272,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
             if (anyd) {
 #pragma unroll
                 for (int u = 0; u < W; ++u)
-                    if (direct[sb[u]]) dl[u] = 0xFFFFFFFFu;  // equality buckets: k_fill
+                    if (direct[sb[u]]) dl[u] = 0xFFFFFFFFu;  // equality buckets: filled after the level
             }
 #pragma unroll
             for (int u = 0; u < W; ++u) {
@@ -1612,7 +1598,11 @@ __device__ __forceinline__ LeafDesc uniform_desc(const LeafDesc& d) {
 }
 
 // Leaves of at most bitonic_cutoff keys: one per wave, four per work-group of
-// kLeafWG threads, each sorted in its wave's registers.
+// kLeafWG threads, each sorted in its wave's registers.  After a partition
+// level the same work-groups then write the level's equality buckets: the
+// splitter value, in caller representation, over each fill task's range of d.
+// This launch follows the level's scatter, so no tile of the level still
+// reads those positions, and no leaf or later level touches them.
 template <int DT>
 __global__ __launch_bounds__(kLeafWG) void k_leaf_small(LeafArgs a) {
     using K = typename KeyOf<DT>::K;
@@ -1638,6 +1628,14 @@ __global__ __launch_bounds__(kLeafWG) void k_leaf_small(LeafArgs a) {
                 [&](uint32_t i, K v) { out[i] = from_key<DT>(v); });
         }
         __syncthreads();  // desc[] is rewritten by the next iteration
+    }
+    if (!list_mode) {
+        const uint32_t nt = min(a.lv.ctl->nfill, a.lv.fill_cap);
+        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+            const FillTask f = a.lv.fill[t];
+            const K v = from_key<DT>(((const K*)a.lv.spl)[f.spl_index]);
+            for (uint32_t i = tid; i < f.n; i += kLeafWG) dout[f.start + i] = v;
+        }
     }
 }
 
@@ -1962,7 +1960,6 @@ static hipError_t level_step_t(int step, const LevelArgs& a, const Grid& g, hipS
             hipLaunchKernelGGL(k_scatter<DT>, dim3(std::max(1, g.part * kPartWG / kScatterWG)), dim3(kScatterWG), 0, st,
                                a);
             break;
-        case 7: hipLaunchKernelGGL(k_fill<DT>, dim3(g.plan), dim3(256), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

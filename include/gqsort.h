@@ -153,7 +153,9 @@ gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
 /* Per-kernel timing with HIP events on the sort's own stream.  While enabled,
  * every launch is bracketed by events; gqs_kernel_times() synchronises and
  * returns, per kernel kind, the number of launches and total milliseconds.
- * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init, 8 fill. */
+ * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init, 8 fill
+ * (kept for the layout of the arrays: equality buckets are filled inside the
+ * leaf launches, so kind 8 reports no launches). */
 gqs_status gqs_set_profiling(gqs_ctx* ctx, int on);
 gqs_status gqs_kernel_times(gqs_ctx* ctx, uint32_t* launches, double* ms, int nkinds);
 /* Per-launch (kind, ms) records collected by the last gqs_kernel_times();
