@@ -728,6 +728,10 @@ template <class K>
 struct TileLds {
     K sK[2 * kSplStride];  // splitters, padded with the maximum key
     uint8_t sLut[kLutCells];
+    // 32-bit keys, search depth >= 1: per cell, the splitter that the first
+    // search step compares with and the cell's first candidate, read by one
+    // 8-byte LDS load instead of two dependent random reads
+    uint2 sLut2[sizeof(K) == 4 ? kLutCells : 1];
     uint16_t sMeta[kSplStride];
     uint32_t hist[kSplStride];
     uint32_t seg;
@@ -746,6 +750,17 @@ __device__ __forceinline__ void load_splitters(const LevelArgs& a, uint32_t s, T
     const uint32_t* lut = reinterpret_cast<const uint32_t*>(a.lut + (size_t)s * kLutCells);
     uint32_t* dl = reinterpret_cast<uint32_t*>(L.sLut);
     for (uint32_t i = threadIdx.x; i < (uint32_t)kLutCells / 4; i += blockDim.x) dl[i] = lut[i];
+    if constexpr (sizeof(K) == 4) {
+        const uint32_t depth = uni(a.plans[s].lut_depth);
+        if (depth >= 1) {     // uniform
+            __syncthreads();  // sK and sLut complete
+            const uint32_t off = (1u << (depth - 1)) - 1u;
+            for (uint32_t c = threadIdx.x; c < (uint32_t)kLutCells; c += blockDim.x) {
+                const uint32_t i0 = L.sLut[c];
+                L.sLut2[c] = make_uint2((uint32_t)L.sK[i0 + off], i0);
+            }
+        }
+    }
 }
 
 // Classification through the lookup table: the table gives the first
@@ -754,14 +769,21 @@ template <class K, int D, bool EQ, int N>
 __device__ __forceinline__ void classify_lut_d(const K (&key)[N], uint32_t (&b)[N], const TileLds<K>& L, K lo,
                                                uint32_t shift) {
     uint32_t i[N];
+    // 32-bit keys: the first step's splitter comes with the candidate
+    constexpr bool kPair = sizeof(K) == 4 && D >= 1;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const K off = key[j] - lo;
         const K cell = (key[j] < lo) ? K(0) : min(off >> shift, (K)(kLutCells - 1));
-        i[j] = L.sLut[(uint32_t)cell];
+        if constexpr (kPair) {
+            const uint2 e = L.sLut2[(uint32_t)cell];
+            i[j] = e.y + ((e.x < (uint32_t)key[j]) ? (1u << (D - 1)) : 0u);
+        } else {
+            i[j] = L.sLut[(uint32_t)cell];
+        }
     }
 #pragma unroll
-    for (int st = D - 1; st >= 0; --st) {
+    for (int st = kPair ? D - 2 : D - 1; st >= 0; --st) {
         const uint32_t step = 1u << st;
 #pragma unroll
         for (int j = 0; j < N; ++j) i[j] += (L.sK[i[j] + step - 1] < key[j]) ? step : 0u;
