@@ -200,6 +200,21 @@ __device__ __forceinline__ void hist_add2(uint32_t* hist, uint32_t b, bool valid
     }
 }
 
+// Whether every valid key of the wave's N slots falls into one bucket (sorted
+// runs, constant input); b0 is that bucket, l0 the first valid lane of slot 0.
+template <int N>
+__device__ __forceinline__ bool wave_one_bucket(const uint32_t (&b)[N], const bool (&valid)[N], uint32_t& b0,
+                                                uint32_t& l0) {
+    const uint64_t v0 = __ballot(valid[0]);
+    if (v0 == 0) return false;
+    l0 = (uint32_t)__builtin_ctzll(v0);
+    b0 = (uint32_t)__builtin_amdgcn_readlane((int)b[0], l0);
+    bool same = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) same = same && (!valid[j] || b[j] == b0);
+    return __ballot(!same) == 0;
+}
+
 // hist_rank for N keys per thread at once: every key's atomic is issued
 // before any result is waited for.  Invalid keys (valid[j] false) take no
 // slot and get rank 0.
@@ -213,6 +228,27 @@ __device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[
             if (valid[j]) r[j] = atomicAdd(&hist[b[j]], 1u);
         }
         return;
+    }
+    {
+        // the whole wave in one bucket: one add for all its keys, ranks in
+        // slot-major, lane-minor order
+        uint32_t b0, l0;
+        if (wave_one_bucket<N>(b, valid, b0, l0)) {
+            uint64_t vm[N];
+            uint32_t before[N], total = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                vm[j] = __ballot(valid[j]);
+                before[j] = total;
+                total += (uint32_t)__popcll(vm[j]);
+            }
+            uint32_t base = 0;
+            if (lane_id() == l0) base = atomicAdd(&hist[b0], total);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, l0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) r[j] = valid[j] ? base + before[j] + mbcnt(vm[j]) : 0u;
+            return;
+        }
     }
     // as hist_add2: a wave of at most two buckets takes one slot range per
     // bucket (rank = range base + lanes below in the same bucket).  All
@@ -902,6 +938,9 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
                 uint32_t bk[TS::kKeysPerThread];
                 classify_lut<K, TS::kKeysPerThread>(keys, bk, L, p);
                 {
+                    // (a one-atomic path for waves in one bucket, as the
+                    // scatter has, raised this kernel to 138 VGPRs, three
+                    // waves per SIMD, and slowed every input)
                     const bool try_peel = clustered(bk[0], valid(sub + tid));
 #pragma unroll
                     for (int j = 0; j < TS::kKeysPerThread; ++j)
