@@ -1359,18 +1359,17 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
 // Local sort of one leaf (the reference's lqsort_kernel,
 // CL12/QuicksortKernels.cl:267-421, with sort_threshold/bitonic_sort as its
 // tail, :40-97), reshaped for wave64:
-//   * leaves of at most kWaveSortMax keys: one wave sorts the leaf in its
-//     registers (lane-major bitonic network, cross-lane steps by shuffles);
-//     the four waves of a work-group take four such leaves at once;
-//   * larger leaves: one k-way partition of the whole leaf by the work-group
-//     (sampled splitters, keys held in registers, ranks from wave-aggregated
-//     LDS atomics, keys placed by bucket in LDS), then every sub-bucket is a
-//     bitonic tail for one wave; equality sub-buckets are filled with their
-//     value; a sub-bucket still above kWaveSortMax (rare) is sorted in LDS by
-//     the whole work-group.
-constexpr int kWaveSortMax = 512;
-constexpr int kLeafSplit = 64;  // largest local fan-out
-
+//   * leaves of at most bitonic_cutoff (<= 512) keys, k_leaf_small: one wave
+//     sorts the leaf in its registers (lane-major bitonic network, cross-lane
+//     steps by DPP and swizzles); the four waves of a work-group take four
+//     such leaves at once;
+//   * larger leaves, k_leaf_big: one interpolation partition of the whole
+//     leaf by the work-group between its minimum and maximum (keys held in
+//     registers, ranks from returning LDS adds, keys placed by local bucket in
+//     LDS), then every thread sorts its local bucket in registers with an
+//     odd-even merge network; buckets above the register capacity go to one
+//     wave each (bitonic) or, above bitonic_cutoff (rare), to the whole
+//     work-group in LDS.
 constexpr int kLeafBigWG = 512;
 // largest bucket one thread sorts in registers (64 VGPRs of keys at most)
 template <class K>
