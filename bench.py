@@ -33,8 +33,8 @@ HBM_PEAK = 8.0e12  # MI355X published HBM3E bandwidth, B/s
 DTYPES = {"u32": gqsort.U32, "f32": gqsort.F32, "f64": gqsort.F64}
 # Algorithmic HBM bytes per key processed, per kernel kind (key bytes kb):
 # count reads each key once, scatter and the local sort read and write it
-# once, fill writes it once.
-ALG_BYTES_PER_KEY = {"count": 1, "scatter": 2, "leaf": 2, "fill": 1}
+# once (equality buckets are filled inside the local-sort launches).
+ALG_BYTES_PER_KEY = {"count": 1, "scatter": 2, "leaf": 2}
 
 
 def log(msg):
@@ -42,8 +42,7 @@ def log(msg):
 
 
 # kernels behind each kind of the library's per-kind timing
-KIND_KERNELS = {"count": ("k_count",), "scatter": ("k_scatter",), "leaf": ("k_leaf_big", "k_leaf_small"),
-                "fill": ("k_fill",)}
+KIND_KERNELS = {"count": ("k_count",), "scatter": ("k_scatter",), "leaf": ("k_leaf_big", "k_leaf_small")}
 
 
 def measured_traffic(kind, workload):
@@ -73,7 +72,7 @@ def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb, workload):
     level_keys = [int(k) for k in st.level_keys]
     keys_partitioned = sum(level_keys)
     # keys handled per kind over the whole sort
-    keys = {"count": keys_partitioned, "scatter": keys_partitioned, "leaf": n, "fill": 0}
+    keys = {"count": keys_partitioned, "scatter": keys_partitioned, "leaf": n}
     dom = max((k for k in ALG_BYTES_PER_KEY if times[k][0]), key=lambda k: times[k][1])
     launches, ms = times[dom]
     alg_bytes_per_launch = ALG_BYTES_PER_KEY[dom] * kb * keys[dom] / launches

@@ -149,8 +149,8 @@ def test_large_leaves_from_a_level(gpu_ctx):
 @pytest.mark.parametrize("distinct", [3, 17, 1000])
 def test_duplicate_heavy(gpu_ctx, dt, distinct):
     """Few distinct values with skewed (Zipf-like) frequencies: repeated
-    splitters give equality buckets that k_fill writes, and waves whose keys
-    share a bucket take the aggregated histogram path."""
+    splitters give equality buckets that the leaf pass fills, and waves whose
+    keys share a bucket take the aggregated histogram and rank paths."""
     n = 3 << 20
     rng = np.random.default_rng(distinct)
     w = 1.0 / np.arange(1, distinct + 1) ** 1.2
