@@ -158,3 +158,37 @@ def test_segmented_random_lengths(gpu_ctx, dtype):
     s = gpu_ctx.stats()
     assert s.overflow == 0 and s.oversize_leaves == 0
     stream.close()
+
+
+@pytest.mark.parametrize("dtype", [gqsort.U32, gqsort.F32, gqsort.F64])
+def test_scatter_tile_blocks(gpu_ctx, dtype):
+    """Arrays of 1 ... 65 partition tiles, exact and off by one: the scatter
+    takes each tile's run starts from the count matrix, loaded in blocks of 32
+    tiles, and work-group ranges cross array boundaries anywhere.  Every third
+    array is sorted ascending and every third constant, so that waves whose
+    keys share one bucket (also in the partial last tile) take the
+    one-add rank path."""
+    tile = 16384 if dtype != gqsort.F64 else 8192
+    lens = []
+    for t in (1, 2, 31, 32, 33, 63, 64, 65):
+        lens += [t * tile - 1, t * tile, t * tile + 1]
+    rng = np.random.default_rng(11 + dtype)
+    parts = []
+    for i, n in enumerate(lens):
+        a = gqsort.generate(dtype, "normal" if dtype != gqsort.U32 else "uniform", 100 + i, n)
+        if i % 3 == 1:
+            a = np.sort(a)
+        elif i % 3 == 2:
+            a = np.full(n, a[rng.integers(n)], dtype=a.dtype)
+        parts.append(a)
+    keys = np.concatenate(parts)
+    offs = np.zeros(len(lens) + 1, np.uint64)
+    offs[1:] = np.cumsum(np.array(lens, np.uint64))
+    print(f"case scatter tile blocks dtype={dtype} total={keys.size}", flush=True)
+    d = gqsort.DeviceBuffer.from_array(gpu_ctx, keys)
+    dn = gqsort.DeviceBuffer(gpu_ctx, keys.nbytes)
+    od = gqsort.DeviceBuffer.from_array(gpu_ctx, offs)
+    gpu_ctx.sort_segmented(dtype, d, dn, od, len(lens), keys.size)
+    check_batch(d.download(keys.dtype, keys.size), keys, offs, f"tile blocks dtype={dtype}")
+    s = gpu_ctx.stats()
+    assert s.overflow == 0
