@@ -297,8 +297,8 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=64 << 30)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the sort shards by sample-sort exchange, which this build does not "
-              "run (see DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: across GPUs the sort would shard by a sample-sort exchange of "
+              "key ranges, which is not implemented; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     dt = DTYPES[args.dtype]
