@@ -21,15 +21,15 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # dtype, generator, n, run length
-    (gqsort.U32, "uniform", 1_000_003, 1 << 16),      # 16 runs (the last ragged), 4 rounds
+    (gqsort.U32, "uniform", 1_000_003, 1 << 16),      # 16 runs (the last ragged), 4 merge rounds
     (gqsort.U32, "uniform", 300_001, 100_000),        # 4 runs, the last of one key
-    (gqsort.U32, "gaussian", 500_000, 200_000),       # 3 runs: the third is copied across in round 1
+    (gqsort.U32, "gaussian", 500_000, 200_000),       # 3 runs: the third is copied across in the first merge round
     (gqsort.U32, "equal", 1 << 20, 1 << 16),          # every comparison a tie
     (gqsort.U32, "sorted", 1 << 20, 3 << 15),
     (gqsort.U32, "reverse", 1 << 20, 3 << 15),
     (gqsort.U32, "bucket", (1 << 20) + 3, 1 << 17),
-    (gqsort.U32, "permutation", 1001, 2),             # runs of two keys: 9 rounds of tiny merges
-    (gqsort.F32, "normal", (1 << 20) + 77, 1 << 18),  # 5 runs, 3 rounds: copied back to d
+    (gqsort.U32, "permutation", 1001, 2),             # runs of two keys: 9 merge rounds of tiny merges
+    (gqsort.F32, "normal", (1 << 20) + 77, 1 << 18),  # 5 runs, 3 merge rounds: copied back to d
     (gqsort.F64, "normal", 700_001, 123_457),         # 6 runs, 3 rounds
     (gqsort.F64, "ties", 262_144, 65_536),            # 4 runs, 2 rounds, 80 distinct values
 ]
