@@ -1380,42 +1380,50 @@ constexpr uint32_t lane_sort_max() { return sizeof(K) == 4 ? 32u : 16u; }
 // unrolling every register index is a constant (a loop whose trip count
 // depends on an outer loop's counter stays a loop, and v[] is then indexed
 // at run time through s_set_gpr_idx).
-template <class K, int N, int P, int D>
+// Pruned to the first M wires (M <= N): when v[M..N) hold the maximum key, a
+// compare-exchange touching one of them leaves both wires as they are (the
+// smaller value is already on the lower wire, and a maximum never moves
+// down), so dropping every such comparator sorts v[0..M) all the same; for
+// N = 32 that is 162, 132 and 103 comparators for M = 28, 24 and 20 instead
+// of 191.
+template <class K, int N, int P, int D, int M>
 __device__ __forceinline__ void oem_step(K (&v)[N]) {
     if constexpr (D >= 1) {
 #pragma unroll
         for (int j = D % P; j + D < N; j += 2 * D)
 #pragma unroll
             for (int i = 0; i < D; ++i)
-                if (i + j + D < N && (i + j) / (2 * P) == (i + j + D) / (2 * P)) {
+                if (i + j + D < M && (i + j) / (2 * P) == (i + j + D) / (2 * P)) {
                     const K lo = kmin(v[i + j], v[i + j + D]), hi = kmax(v[i + j], v[i + j + D]);
                     v[i + j] = lo;
                     v[i + j + D] = hi;
                 }
-        oem_step<K, N, P, D / 2>(v);
+        oem_step<K, N, P, D / 2, M>(v);
     }
 }
-template <class K, int N, int P>
+template <class K, int N, int P, int M>
 __device__ __forceinline__ void oem_merge(K (&v)[N]) {
     if constexpr (P < N) {
-        oem_step<K, N, P, P>(v);
-        oem_merge<K, N, 2 * P>(v);
+        oem_step<K, N, P, P, M>(v);
+        oem_merge<K, N, 2 * P, M>(v);
     }
 }
-template <class K, int N>
+template <class K, int N, int M = N>
 __device__ __forceinline__ void oem_network(K (&v)[N]) {
-    oem_merge<K, N, 1>(v);
+    static_assert(M <= N, "pruned to at most N wires");
+    oem_merge<K, N, 1, M>(v);
 }
 
-// One thread sorts A[lo, lo + c) (c <= N) in registers.
-template <class K, int N>
+// One thread sorts A[lo, lo + c) (c <= M <= N) in registers, with the
+// N-wire network pruned to M wires.
+template <class K, int N, int M = N>
 __device__ __forceinline__ void lane_sort(K* A, uint32_t lo, uint32_t c) {
     K v[N];
 #pragma unroll
-    for (int r = 0; r < N; ++r) v[r] = ((uint32_t)r < c) ? A[lo + r] : key_max<K>();
-    oem_network<K, N>(v);
+    for (int r = 0; r < N; ++r) v[r] = (r < M && (uint32_t)r < c) ? A[lo + r] : key_max<K>();
+    oem_network<K, N, M>(v);
 #pragma unroll
-    for (int r = 0; r < N; ++r)
+    for (int r = 0; r < M; ++r)
         if ((uint32_t)r < c) A[lo + r] = v[r];
 }
 
@@ -1954,13 +1962,20 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
 #pragma unroll
             for (int dd = 32; dd >= 1; dd >>= 1) wm = max(wm, (uint32_t)__shfl_xor(wm, dd, 64));
             const uint32_t c = fits ? mycnt : 0u;
+            // network pruned to the wave's largest bucket (rounded up to a
+            // multiple of 4 above 16 keys)
             if constexpr (sizeof(K) == 4) {
-                if (wm > 16) lane_sort<K, 32>(A, mystart, c);
-                else if (wm > 8) lane_sort<K, 16>(A, mystart, c);
+                if (wm > 28) lane_sort<K, 32>(A, mystart, c);
+                else if (wm > 24) lane_sort<K, 32, 28>(A, mystart, c);
+                else if (wm > 20) lane_sort<K, 32, 24>(A, mystart, c);
+                else if (wm > 16) lane_sort<K, 32, 20>(A, mystart, c);
+                else if (wm > 12) lane_sort<K, 16>(A, mystart, c);
+                else if (wm > 8) lane_sort<K, 16, 12>(A, mystart, c);
                 else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
                 else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
             } else {
-                if (wm > 8) lane_sort<K, 16>(A, mystart, c);
+                if (wm > 12) lane_sort<K, 16>(A, mystart, c);
+                else if (wm > 8) lane_sort<K, 16, 12>(A, mystart, c);
                 else if (wm > 4) lane_sort<K, 8>(A, mystart, c);
                 else if (wm > 1) lane_sort<K, 4>(A, mystart, c);
             }
