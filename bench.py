@@ -88,23 +88,80 @@ def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb, workload):
     }, {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in times.items() if v[0]}, level_keys
 
 
-def cpu_baseline(dt, gen, seed, sample_n, threads):
+def cpu_info():
+    """Host provenance: nproc and the CPU model string."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(host, gpu_out, threads):
+    """std::sort on one core and a parallel sort on `threads` cores of the same
+    input the GPU sorted (the reference's CPU legs, CL12/QuicksortMain.cpp:
+    418-437), each timed around the sort call alone; the GPU result must equal
+    std::sort's bit for bit."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
-    a = gqsort.generate(dt, gen, seed, sample_n)
+    lib = orc.load()
+    suffix = orc._SUFFIX[host.dtype]
+    n = host.size
+    a = host.copy()
     t0 = time.perf_counter()
-    orc.std_sort(a)
+    getattr(lib, f"orc_std_sort_{suffix}")(a.ctypes.data, n)
     t1 = time.perf_counter()
-    par = None
-    if threads > 1:
-        t2 = time.perf_counter()
-        orc.par_sort(a, threads)
-        t3 = time.perf_counter()
-        par = {"value": round(sample_n / (t3 - t2) / 1e6, 2), "unit": "Mkeys/s", "cores": threads,
-               "kind": "port", "sample": f"{sample_n} keys, chunked std::sort + merge (tbb::parallel_sort stand-in)"}
-    return {"value": round(sample_n / (t1 - t0) / 1e6, 2), "unit": "Mkeys/s", "cores": 1, "kind": "port",
-            "sample": f"{sample_n} {gen} keys (seed {seed}), std::sort on one core"}, par
+    exact = gpu_out is not None and orc.mismatch(gpu_out, a)[0] == 0
+    b = host.copy()
+    t2 = time.perf_counter()
+    getattr(lib, f"orc_par_sort_{suffix}")(b.ctypes.data, n, threads)
+    t3 = time.perf_counter()
+    par_exact = orc.mismatch(b, a)[0] == 0
+    sample = f"the timed input itself ({n} keys)"
+    return ({"value": round(n / (t1 - t0) / 1e6, 2), "unit": "Mkeys/s", "cores": 1, "kind": "port",
+             "sample": sample + ", std::sort on one core", "ms": round((t1 - t0) * 1e3, 1),
+             "gpu_equals_std_sort": exact},
+            {"value": round(n / (t3 - t2) / 1e6, 2), "unit": "Mkeys/s", "cores": threads, "kind": "port",
+             "sample": sample + f", {threads} threads: chunked std::sort + pairwise merges "
+                                "(tbb::parallel_sort stand-in: TBB headers are absent)",
+             "ms": round((t3 - t2) * 1e3, 1), "equals_std_sort": par_exact})
+
+
+def host_pointer_e2e(ctx, dt, host, reps):
+    """The reference's timed call: GPUQSort on host arrays, upload + sort +
+    download (CL12/QuicksortMain.cpp:491-494), wall clock per call."""
+    d = host.copy()
+    dn = np.empty_like(d)
+    ts = []
+    for r in range(reps + 1):
+        np.copyto(d, host)
+        t0 = time.perf_counter()
+        gqsort.GPUQSort(ctx, d.size, d, dn)
+        t1 = time.perf_counter()
+        if r:
+            ts.append((t1 - t0) * 1e3)
+    ok = bool(np.all(d[1:] >= d[:-1])) if d.dtype == np.uint32 else bool(np.all(np.diff(d) >= 0))
+    ms = float(np.median(ts))
+    return {"ms": round(ms, 2), "Mkeys_s": round(d.size / ms / 1e3, 1),
+            "GB_s_moved": round(2 * d.nbytes / (ms / 1e3) / 1e9, 1), "reps": reps, "sorted": ok,
+            "note": "pageable host buffers staged through pinned chunks; includes PCIe both ways"}
+
+
+def multiset_ok(inp, out):
+    """Order-independent checksums of the key multiset (sum and xor of the bit
+    patterns) plus sortedness."""
+    iv = inp.view(np.uint32 if inp.itemsize == 4 else np.uint64)
+    ov = out.view(iv.dtype)
+    sorted_ok = bool(np.all(out[1:] >= out[:-1])) if inp.dtype == np.uint32 else bool(
+        np.all(np.diff(out.astype(np.float64)) >= 0))
+    sums_ok = int(iv.sum(dtype=np.uint64)) == int(ov.sum(dtype=np.uint64)) and \
+        int(np.bitwise_xor.reduce(iv)) == int(np.bitwise_xor.reduce(ov))
+    return sorted_ok, sums_ok
 
 
 def bench_batched(ctx, args, stream, ev0, ev1):
@@ -264,12 +321,12 @@ def bench_inputs(ctx, stream, ev0, ev1, d, dn, pristine, cases, steps):
             if r:
                 ts.append(ev0.elapsed_ms(ev1))
         res = d.download(host.dtype, n, stream)
-        ok = bool(np.all(res[1:] >= res[:-1])) if dname == "u32" else bool(
-            np.all(np.diff(res.astype(np.float64)) >= 0))
+        sorted_ok, sums_ok = multiset_ok(host, res)
         ms = float(np.median(ts))
         out.append({"dtype": dname, "generator": gen, "n": n, "ms": round(ms, 4),
                     "Mkeys_s": round(n / ms / 1e3, 1),
-                    "roofline_frac": round(2 * n * kb / (ms / 1e3) / HBM_PEAK, 4), "sorted": ok})
+                    "roofline_frac": round(2 * n * kb / (ms / 1e3) / HBM_PEAK, 4), "sorted": sorted_ok,
+                    "multiset_checksums": sums_ok})
     return out
 
 
@@ -282,7 +339,8 @@ def main():
     ap.add_argument("--gen", default="uniform")
     ap.add_argument("--dtype", default="u32", choices=sorted(DTYPES))
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--no-host", action="store_true", help="skip the host-pointer end-to-end timing")
+    ap.add_argument("--host-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -310,6 +368,7 @@ def main():
 
     ctx = gqsort.Context(0)
     info = ctx.device_info()
+    info["report"] = ctx.device_report()
     stream = gqsort.Stream(ctx)
     pristine = gqsort.DeviceBuffer.from_array(ctx, host, stream)
     d = gqsort.DeviceBuffer(ctx, nbytes)
@@ -337,14 +396,10 @@ def main():
 
     # correctness of the last step: sortedness and multiset checksums (size-independent)
     check = None
+    out = None
     if not args.no_check:
         out = d.download(host.dtype, n, stream)
-        iv = host.view(np.uint32 if kb == 4 else np.uint64)
-        ov = out.view(iv.dtype)
-        sorted_ok = bool(np.all(out[1:] >= out[:-1])) if args.dtype == "u32" else bool(
-            np.all(np.diff(out.astype(np.float64)) >= 0))
-        sums_ok = int(iv.sum(dtype=np.uint64)) == int(ov.sum(dtype=np.uint64)) and \
-            int(np.bitwise_xor.reduce(iv)) == int(np.bitwise_xor.reduce(ov))
+        sorted_ok, sums_ok = multiset_ok(host, out)
         check = {"sorted": sorted_ok, "multiset_checksums": sums_ok}
         if not (sorted_ok and sums_ok):
             log(f"CHECK FAILED: {check}")
@@ -356,9 +411,19 @@ def main():
 
     cpu = par = None
     if not args.no_cpu:
+        # the box's CPU share for one GPU is 16 cores, whatever nproc reports
         threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on a {args.cpu_sample}-key sample")
-        cpu, par = cpu_baseline(dt, args.gen, args.seed, args.cpu_sample, threads)
+        log(f"cpu baselines on the same {n}-key input (1 and {threads} threads)")
+        cpu, par = cpu_baseline(host, out, threads)
+        if out is not None and not cpu["gpu_equals_std_sort"]:
+            log("CHECK FAILED: GPU output differs from std::sort")
+            sys.exit(1)
+    out = None
+
+    e2e = None
+    if not args.no_host:
+        log("host-pointer end-to-end (upload + sort + download)")
+        e2e = host_pointer_e2e(ctx, dt, host, args.host_reps)
 
     inputs = None
     if not args.no_inputs and n == 1 << 28 and args.dtype == "u32":
@@ -367,7 +432,7 @@ def main():
         cases = [("u32", g, n) for g in ("gaussian", "sorted", "reverse", "equal", "bucket")]
         cases += [("f32", "normal", n >> 1), ("f64", "normal", n >> 2)]
         inputs = bench_inputs(ctx, stream, ev0, ev1, d, dn, pristine, cases, args.input_steps)
-        if not all(c["sorted"] for c in inputs):
+        if not all(c["sorted"] and c["multiset_checksums"] for c in inputs):
             log(f"CHECK FAILED: {inputs}")
             sys.exit(1)
 
@@ -401,8 +466,13 @@ def main():
                    "parallelism": "single-gpu"},
         "sort_roofline_frac": round(sort_frac, 4),
         "roofline": roof,
+        "roofline_sort": {"bound": "hbm", "achieved": round(2 * nbytes / (ms_per_step / 1e3) / 1e9, 1),
+                          "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(sort_frac, 4),
+                          "traffic": None, "what": "whole sort: 2*N*sizeof(key) / time per sort"},
         "cpu_baseline": cpu,
         "cpu_parallel": par,
+        "host": cpu_info(),
+        "host_pointer_e2e": e2e,
         "kernels_ms": kernels,
         "level_keys": level_keys,
         "step_ms": {"min": round(min(times), 4), "median": round(float(np.median(times)), 4),

@@ -8,13 +8,18 @@ Python binding that loads it.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "gqsort", "libgqsort.so")
 ARCH = os.environ.get("GQS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "merge.hip", "gqsort.cpp", "devutil.cpp", "generate.cpp", "serve.cpp", "autotune.cpp"]
+# (source, extra defines): kernels_typed.hip is compiled once per key type.
+UNITS = [("kernels.hip", []), ("kernels_typed.hip", ["-DGQS_DT=0"]), ("kernels_typed.hip", ["-DGQS_DT=1"]),
+         ("kernels_typed.hip", ["-DGQS_DT=2"]), ("merge.hip", []), ("gqsort.cpp", []), ("devutil.cpp", []),
+         ("generate.cpp", []), ("serve.cpp", []), ("autotune.cpp", [])]
+OBJ_DIR = os.path.join(HERE, "_build")
 
 # Packed FP32 instructions are disabled for device code; the host pass of
 # hipcc warns that it does not know the feature, which is harmless.
@@ -28,17 +33,39 @@ def hipcc():
     return "hipcc"
 
 
+def _compile(unit, verbose):
+    src, defs = unit
+    tag = "".join(d.split("=")[-1] for d in defs)
+    obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + (f"_{tag}" if tag else "") + ".o")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-pthread", "-c",
+           "-Wno-unused-command-line-argument", "-Wno-unused-value", *DEVICE_FLAGS, *defs,
+           "-I", os.path.join(HERE, "..", "include"), "-o", obj, os.path.join(CSRC, src)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return None, r.stdout + r.stderr
+    return obj, ""
+
+
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
-           "-Wno-unused-command-line-argument", *DEVICE_FLAGS,
-           "-I", os.path.join(HERE, "..", "include"), "-o", OUT + ".tmp", *srcs]
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    units = [u for u in UNITS if os.path.exists(os.path.join(CSRC, u[0]))]
+    jobs = max(1, min(len(units), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda u: _compile(u, verbose), units))
+    errs = [msg for obj, msg in results if obj is None]
+    if errs:
+        sys.stderr.write("\n".join(errs))
+        raise RuntimeError("hipcc failed building libgqsort.so")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", OUT + ".tmp",
+           *[obj for obj, _ in results]]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libgqsort.so")
+        raise RuntimeError("hipcc failed linking libgqsort.so")
     os.replace(OUT + ".tmp", OUT)
     build_cli(verbose)
     return OUT

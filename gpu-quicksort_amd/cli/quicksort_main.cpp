@@ -162,13 +162,31 @@ int big_test(gqs_ctx* ctx, size_t n, unsigned iterations, const std::string& dev
     return failures ? 2 : 0;
 }
 
-// The reference's show_CL report (CL12/OpenCLUtils.h:291-387), for the HIP device.
+// The reference's show_CL report (QueryPrintOpenCLDeviceInfo,
+// CL12/OpenCLUtils.h:291-387): one line per property, for the HIP device.
 void show_device(gqs_ctx* ctx) {
-    int cus = 0, lds = 0, wave = 0;
-    char name[256] = {0};
-    if (gqs_device_info(ctx, &cus, &lds, &wave, name, sizeof name) != GQS_OK) return;
-    printf("Device: %s\n  compute units: %d\n  LDS per work-group: %d bytes\n  wavefront size: %d\n", name, cus, lds,
-           wave);
+    gqs_device_props p{};
+    if (gqs_device_report(ctx, &p) != GQS_OK) return;
+    printf("Using platform: AMD ROCm/HIP and device: %s.\n", p.name);
+    printf("HIP Device info:\n");
+    printf("DEVICE_NAME                 :%s\n", p.name);
+    printf("DEVICE_ARCH                 :%s\n", p.arch);
+    printf("PCI_BUS_ID                  :%s\n", p.pci_bus_id);
+    printf("DRIVER_VERSION              :%d\n", p.driver_version);
+    printf("RUNTIME_VERSION             :%d\n", p.runtime_version);
+    printf("MAX_COMPUTE_UNITS           :%8d\n", p.compute_units);
+    printf("WAVEFRONT_SIZE              :%8d\n", p.wave_size);
+    printf("MAX_WORK_ITEM_SIZES         :    (%5d, %5d, %5d)\n", p.max_block_dim[0], p.max_block_dim[1],
+           p.max_block_dim[2]);
+    printf("MAX_WORK_GROUP_SIZE         :%8d\n", p.max_threads_per_block);
+    printf("MAX_CLOCK_FREQUENCY (MHz)   :%8d\n", p.clock_khz / 1000);
+    printf("MEMORY_CLOCK (MHz)          :%8d\n", p.mem_clock_khz / 1000);
+    printf("MEMORY_BUS_WIDTH (bits)     :%8d\n", p.mem_bus_bits);
+    printf("PEAK_MEMORY_BANDWIDTH (GB/s):%12.1f\n", p.hbm_peak_gbs);
+    printf("LOCAL_MEM_SIZE (per WG)     :%12.1f\n", (double)p.lds_per_block);
+    printf("LOCAL_MEM_SIZE (per CU)     :%12.1f\n", (double)p.lds_per_cu);
+    printf("L2_CACHE_SIZE               :%12.1f\n", (double)p.l2_bytes);
+    printf("GLOBAL_MEM_SIZE             :%12.1f\n", (double)p.global_mem_bytes);
 }
 
 }  // namespace

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -168,14 +169,15 @@ gqs_status gqs_tune_save(gqs_ctx* c, const char* path) {
 }
 
 // Reads the format gqs_tune_save writes: one JSON object per entry line with
-// integer fields (unknown fields keep their defaults).
+// integer fields (unknown fields keep their defaults).  Every entry must name
+// a known key type and pass the range checks of gqs_set_params; otherwise the
+// file is rejected as a whole and the context's table is left unchanged.
 gqs_status gqs_tune_load(gqs_ctx* c, const char* path) {
     if (!c || !path) return GQS_EINVAL;
     std::ifstream f(path);
     if (!f) return c->set_error(GQS_EINVAL, "gqs_tune_load: cannot open file");
     std::lock_guard<std::mutex> lk(c->mu);
     std::string line;
-    int loaded = 0;
     auto field = [](const std::string& l, const std::string& name, double* out) {
         const std::string key = "\"" + name + "\":";
         const size_t at = l.find(key);
@@ -183,20 +185,42 @@ gqs_status gqs_tune_load(gqs_ctx* c, const char* path) {
         *out = std::strtod(l.c_str() + at + key.size(), nullptr);
         return true;
     };
+    std::map<std::pair<int, int>, gqs_params> table;
+    std::map<std::pair<int, int>, double> table_ms;
+    int lineno = 0;
     while (std::getline(f, line)) {
+        ++lineno;
         double dt, lg, v;
         if (!field(line, "dtype", &dt) || !field(line, "log2n", &lg)) continue;
+        char msg[128];
+        if (dt != GQS_U32 && dt != GQS_F32 && dt != GQS_F64) {
+            std::snprintf(msg, sizeof msg, "gqs_tune_load: line %d: unknown dtype", lineno);
+            return c->set_error(GQS_EINVAL, msg);
+        }
+        if (lg < 1 || lg > 63) {
+            std::snprintf(msg, sizeof msg, "gqs_tune_load: line %d: log2n out of range", lineno);
+            return c->set_error(GQS_EINVAL, msg);
+        }
         gqs_params p{};
         gqs_default_params((int)dt, &p);
         if (field(line, "mode", &v)) p.mode = (int32_t)v;
+        bool in_range = true;
         for (const Field& fd : kFields)
-            if (field(line, fd.name, &v)) p.*fd.ptr = (uint32_t)v;
+            if (field(line, fd.name, &v)) {
+                in_range = in_range && v >= 0 && v <= 4294967295.0;
+                p.*fd.ptr = (uint32_t)v;
+            }
+        if (!in_range || !gqs::params_valid(p)) {
+            std::snprintf(msg, sizeof msg, "gqs_tune_load: line %d: parameter out of range", lineno);
+            return c->set_error(GQS_EINVAL, msg);
+        }
         const auto key = std::make_pair((int)dt, (int)lg);
-        c->tuned[key] = p;
-        if (field(line, "ms", &v)) c->tuned_ms[key] = v;
-        ++loaded;
+        table[key] = p;
+        if (field(line, "ms", &v)) table_ms[key] = v;
     }
-    if (!loaded) return c->set_error(GQS_EINVAL, "gqs_tune_load: no entries");
+    if (table.empty()) return c->set_error(GQS_EINVAL, "gqs_tune_load: no entries");
+    for (const auto& kv : table) c->tuned[kv.first] = kv.second;
+    for (const auto& kv : table_ms) c->tuned_ms[kv.first] = kv.second;
     c->use_tuned = true;
     return GQS_OK;
 }

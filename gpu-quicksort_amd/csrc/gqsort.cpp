@@ -69,7 +69,31 @@ void Workspace::release() {
     hipFree(fill);
     hipFree(ctl);
     hipFree(leaves);
+    hipEvent_t ev = done;
+    hipStream_t st = last;
     *this = Workspace{};
+    done = ev;
+    last = st;
+}
+
+void Workspace::release_all() {
+    release();
+    if (done) hipEventDestroy(done);
+    done = nullptr;
+    last = nullptr;
+}
+
+gqs_status Ctx::stream_enter(Workspace& w, hipStream_t st) {
+    if (w.done && w.last != st && fail(hipStreamWaitEvent(st, w.done, 0), "stream order")) return GQS_EDEVICE;
+    return GQS_OK;
+}
+
+gqs_status Ctx::stream_leave(Workspace& w, hipStream_t st) {
+    if (!w.done && fail(hipEventCreateWithFlags(&w.done, hipEventDisableTiming), "stream-order event"))
+        return GQS_EDEVICE;
+    if (fail(hipEventRecord(w.done, st), "stream-order event")) return GQS_EDEVICE;
+    w.last = st;
+    return GQS_OK;
 }
 
 // Capacities from provable bounds (n = total keys, S = segments per level):
@@ -150,7 +174,8 @@ gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
         c.leaves = std::max(c.leaves, ws.cap.leaves);
         c.fill = std::max(c.fill, ws.cap.fill);
         c.huge = std::max(c.huge, ws.cap.huge);
-        hipStreamSynchronize(last_stream);
+        // earlier calls on any stream may still use the buffers
+        hipDeviceSynchronize();
         ws.release();
     }
     // Grow with headroom so that nearby sizes reuse the workspace.
@@ -170,7 +195,7 @@ gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
     if (!e) e = dalloc(&ws.part, c.part);
     if (!e) e = dalloc(&ws.bkt, c.bkt);
     if (!e) e = dalloc(&ws.fill, c.fill);
-    if (!e) e = dalloc(&ws.ctl, (size_t)kMaxLevels + 1);
+    if (!e) e = dalloc(&ws.ctl, (size_t)kCtlBlocks);
     if (!e) e = dalloc(&ws.leaves, c.leaves);
     if (!e) e = dalloc(&ws.huge, c.huge);
     if (e) {
@@ -243,11 +268,10 @@ uint32_t levels_needed(uint64_t n, const gqs_params& p) {
 // The whole sort as one stream-ordered chain.  Either a single key range
 // [0, n) of d (offsets == nullptr) or nseg arrays delimited by offsets.
 gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
-                    uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing) {
+                    uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing, bool keep_status) {
     gqs_params p = run_params(dtype, n);
     if (capturing) p.host_driven = 0;
     if (gqs_status s = ensure_workspace(ws, n, nseg_in, p, dtype)) return s;
-    if (!capturing) last_stream = st;
     capturing_now = capturing;
     const bool prof_on = profiling && !capturing;
     if (prof_on) {
@@ -262,9 +286,9 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     if (offsets) {
         L = levels_needed(n, p);
         prof_begin(7, st);
-        if (fail(launch_init(ws.ctl, kMaxLevels, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
-        if (fail(launch_seed_segments(offsets, nseg_in, nseg_dev, ws.segs[0], &ws.ctl[0], ws.leaves,
-                                      &ws.ctl[0].nleaf_list, p.leaf_max, leaf_cap(dtype), ws.huge, st),
+        if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
+        if (fail(launch_seed_segments(offsets, nseg_in, nseg_dev, n, ws.segs[0], &ws.ctl[0], &ws.ctl[kSeedCtl],
+                                      ws.leaves, p.leaf_max, leaf_cap(dtype), ws.huge, st),
                  "seed"))
             return GQS_EDEVICE;
         prof_end(st);
@@ -273,8 +297,8 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         const bool as_huge = as_leaf && n > leaf_cap(dtype);
         L = as_leaf ? 0 : levels_needed(n, p);
         prof_begin(7, st);
-        if (fail(launch_init(ws.ctl, kMaxLevels, as_huge ? ws.huge : (as_leaf ? ws.leaves : ws.segs[0]), 1, n,
-                             SEG_RAW | (as_huge ? kInitHuge : (as_leaf ? kInitLeaf : 0u)), st),
+        if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, as_huge ? ws.huge : (as_leaf ? ws.leaves : ws.segs[0]),
+                             1, n, SEG_RAW | (as_huge ? kInitHuge : (as_leaf ? kInitLeaf : 0u)), st),
                  "init"))
             return GQS_EDEVICE;
         prof_end(st);
@@ -297,6 +321,8 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     a.oversample = p.oversample;
     a.max_fanout = p.max_fanout;
     a.seg_cap = (uint32_t)ws.cap.seg;
+    if (queue_limit && !capturing) a.seg_cap = std::min(a.seg_cap, queue_limit);
+    a.status = &ws.ctl[kSeedCtl].overflow;
     a.mat_cap = (uint32_t)ws.cap.mat;
     a.bkt_cap = (uint32_t)ws.cap.bkt;
     a.seed = 0x5EEDu;
@@ -310,13 +336,13 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         la.d = d;
         la.dn = dn;
         la.list = ws.leaves;
-        la.list_count = &ws.ctl[0].nleaf_list;
+        la.list_count = &ws.ctl[kSeedCtl].nleaf_list;
         la.lv = a;
-        la.lv.ctl = &ws.ctl[0];
+        la.lv.ctl = &ws.ctl[kSeedCtl];
         la.leaf_max = p.leaf_max;
         la.leaf_cap = leaf_cap(dtype);
         la.huge = ws.huge;
-        la.huge_count = &ws.ctl[0].nhuge;
+        la.huge_count = &ws.ctl[kSeedCtl].nhuge;
         la.is_last = 1;
         la.sort_threshold = p.sort_threshold;
         la.bitonic_cutoff = p.bitonic_cutoff;
@@ -367,25 +393,44 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     return GQS_OK;
 }
 
-gqs_status Ctx::fetch_stats() {
-    if (!ws.ctl) return GQS_OK;
-    std::vector<LevelCtl> c(kMaxLevels);
-    if (fail(hipStreamSynchronize(last_stream), "sync") ||
-        fail(hipMemcpy(c.data(), ws.ctl, sizeof(LevelCtl) * kMaxLevels, hipMemcpyDeviceToHost), "stats readback"))
+gqs_status Ctx::fetch_stats(Workspace& w, gqs_stats* out, uint32_t* bits) {
+    *bits = 0;
+    if (!w.ctl) return GQS_OK;
+    std::vector<LevelCtl> c(kCtlBlocks);
+    if ((w.done && fail(hipEventSynchronize(w.done), "sync")) ||
+        fail(hipMemcpy(c.data(), w.ctl, sizeof(LevelCtl) * kCtlBlocks, hipMemcpyDeviceToHost), "stats readback"))
         return GQS_EDEVICE;
-    uint32_t used = 0, over = 0, ovf = 0;
+    uint32_t used = 0, over = c[kSeedCtl].oversize, ovf = 0;
     for (int i = 0; i < kMaxLevels; ++i) {
         if (c[i].nseg) used = i + 1;
         over += c[i].oversize;
         ovf |= c[i].overflow;
         if (i < 8) {
-            stats.segments[i] = c[i].nseg;
-            stats.level_keys[i] = ((uint64_t)c[i].keys_hi << 32) | c[i].keys_lo;
+            out->segments[i] = c[i].nseg;
+            out->level_keys[i] = ((uint64_t)c[i].keys_hi << 32) | c[i].keys_lo;
         }
     }
-    stats.levels_used = std::max(stats.levels_used, used);
-    stats.oversize_leaves = over;
-    stats.overflow = ovf;
+    out->levels_used = std::max(out->levels_used, used);
+    out->oversize_leaves = over;
+    *bits = c[kSeedCtl].overflow | ovf;
+    out->overflow = *bits;
+    return GQS_OK;
+}
+
+gqs_status Ctx::fetch_stats() { return fetch_stats(ws, &stats, &status_bits); }
+
+gqs_status Ctx::status_of(uint32_t bits, const char* what) {
+    char buf[256];
+    if (bits & (kBadOffsets | kBadCount)) {
+        std::snprintf(buf, sizeof buf, "%s: %s", what,
+                      (bits & kBadOffsets) ? "an array's offsets decrease or pass total_keys (array skipped)"
+                                           : "more arrays than the graph was created for (extra arrays skipped)");
+        return set_error(GQS_EINVAL, buf);
+    }
+    if (bits & kOvfCapacity) {
+        std::snprintf(buf, sizeof buf, "%s: a queue bound was exceeded (flags 0x%x); keys left unsorted", what, bits);
+        return set_error(GQS_EOVERFLOW, buf);
+    }
     return GQS_OK;
 }
 
@@ -444,7 +489,7 @@ gqs_status gqs_destroy(gqs_ctx* c) {
             hipEventDestroy(r.a);
             hipEventDestroy(r.b);
         }
-        c->ws.release();
+        c->ws.release_all();
         c->release_lanes();
         hipFree(c->host_d);
         hipFree(c->host_dn);
@@ -462,19 +507,45 @@ gqs_status gqs_get_params(const gqs_ctx* c, gqs_params* out) {
     return GQS_OK;
 }
 
+gqs_status gqs_check_params(const gqs_params* p) { return (p && gqs::params_valid(*p)) ? GQS_OK : GQS_EINVAL; }
+
 gqs_status gqs_set_params(gqs_ctx* c, const gqs_params* p) {
     if (!c || !p) return GQS_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
-    const bool pow2 = p->max_fanout >= 2 && p->max_fanout <= 256 && (p->max_fanout & (p->max_fanout - 1)) == 0;
-    if ((p->mode != GQS_MODE_MULTIPIVOT && p->mode != GQS_MODE_CLASSIC) || p->leaf_max < 16 ||
-        p->leaf_max > 2 * leaf_cap(DT_U32) || p->leaf_target < 1 || p->oversample < 1 || !pow2 || p->max_levels < 1 || p->max_levels > kMaxLevels - 1 ||
-        p->part_wgs_per_cu < 1 || p->part_wgs_per_cu > 32 || p->leaf_wgs_per_cu < 1 || p->leaf_wgs_per_cu > 32 ||
-        p->sort_threshold < 4 || p->sort_threshold > 4096 || p->bitonic_cutoff < 64 || p->bitonic_cutoff > 512)
-        return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");
+    if (!gqs::params_valid(*p)) return c->set_error(GQS_EINVAL, "gqs_set_params: parameter out of range");
     c->params = *p;
     c->use_tuned = false;
     return GQS_OK;
 }
+
+}  // extern "C"
+
+namespace gqs {
+
+gqs_status Ctx::sort_device_locked(int dtype, void* d, void* dn, uint64_t n, hipStream_t st) {
+    if (fail(hipSetDevice(device), "hipSetDevice")) return GQS_EDEVICE;
+    if (gqs_status s = stream_enter(ws, st)) return s;
+    gqs_status s;
+    if (n >= kMaxRunKeys || (large_run && n > large_run))
+        s = sort_large(dtype, d, dn, n, large_run ? large_run : kLargeRunKeys, st);
+    else
+        s = run(ws, dtype, d, dn, n, nullptr, 0, nullptr, st, false);
+    if (gqs_status t = stream_leave(ws, st)) return s ? s : t;
+    return s;
+}
+
+bool params_valid(const gqs_params& p) {
+    const bool pow2 = p.max_fanout >= 2 && p.max_fanout <= 256 && (p.max_fanout & (p.max_fanout - 1)) == 0;
+    return (p.mode == GQS_MODE_MULTIPIVOT || p.mode == GQS_MODE_CLASSIC) && p.leaf_max >= 16 &&
+           p.leaf_max <= 2 * leaf_cap(DT_U32) && p.leaf_target >= 1 && p.oversample >= 1 && pow2 &&
+           p.max_levels >= 1 && p.max_levels <= (uint32_t)kMaxLevels - 1 && p.host_driven <= 1 &&
+           p.part_wgs_per_cu >= 1 && p.part_wgs_per_cu <= 32 && p.leaf_wgs_per_cu >= 1 && p.leaf_wgs_per_cu <= 32 &&
+           p.sort_threshold >= 4 && p.sort_threshold <= 4096 && p.bitonic_cutoff >= 64 && p.bitonic_cutoff <= 512;
+}
+
+}  // namespace gqs
+
+extern "C" {
 
 gqs_status gqs_sort_device(gqs_ctx* c, int dtype, void* d, void* dn, size_t n, void* stream) {
     if (!c) return GQS_EINVAL;
@@ -482,10 +553,14 @@ gqs_status gqs_sort_device(gqs_ctx* c, int dtype, void* d, void* dn, size_t n, v
     if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_sort_device: bad dtype");
     if (n < 2) return GQS_OK;
     if (!d || !dn) return c->set_error(GQS_EINVAL, "gqs_sort_device: null buffer");
-    if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
-    if (n >= gqs::kMaxRunKeys || (c->large_run && n > c->large_run))
-        return c->sort_large(dtype, d, dn, n, c->large_run ? c->large_run : gqs::kLargeRunKeys, (hipStream_t)stream);
-    return c->run(c->ws, dtype, d, dn, n, nullptr, 0, nullptr, (hipStream_t)stream, false);
+    return c->sort_device_locked(dtype, d, dn, n, (hipStream_t)stream);
+}
+
+gqs_status gqs_set_queue_limit(gqs_ctx* c, uint32_t segments) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->queue_limit = segments;
+    return GQS_OK;
 }
 
 gqs_status gqs_set_large_run(gqs_ctx* c, size_t keys) {
@@ -510,7 +585,8 @@ gqs_status Ctx::sort_large(int dtype, void* d, void* dn, uint64_t n, uint64_t ru
     for (uint64_t off = 0; off < n; off += run_keys) {
         const uint64_t m = std::min(run_keys, n - off);
         if (m < 2) continue;
-        if (gqs_status s = run(ws, dtype, d0 + off * kb, dn0 + off * kb, m, nullptr, 0, nullptr, st, false)) return s;
+        if (gqs_status s = run(ws, dtype, d0 + off * kb, dn0 + off * kb, m, nullptr, 0, nullptr, st, false, off > 0))
+            return s;
     }
     const uint64_t need = merge_split_entries(dtype, n);
     if (need > merge_split_cap) {
@@ -554,7 +630,11 @@ gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const ui
     if (total_keys >= gqs::kMaxRunKeys)
         return c->set_error(GQS_EINVAL, "gqs_sort_segmented: total_keys must be < 2^32 - 2^16");
     if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
-    return c->run(c->ws, dtype, d, dn, total_keys, offsets, (uint32_t)nseg, nullptr, (hipStream_t)stream, false);
+    hipStream_t st = (hipStream_t)stream;
+    if (gqs_status s = c->stream_enter(c->ws, st)) return s;
+    gqs_status s = c->run(c->ws, dtype, d, dn, total_keys, offsets, (uint32_t)nseg, nullptr, st, false);
+    if (gqs_status t = c->stream_leave(c->ws, st)) return s ? s : t;
+    return s;
 }
 
 namespace gqs {
@@ -656,29 +736,144 @@ gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
     if (n < 2) return GQS_OK;
     if (!d || !dn) return c->set_error(GQS_EINVAL, "gqs_sort: null buffer");
     const size_t bytes = n * key_bytes(dtype);
-    {
-        std::lock_guard<std::mutex> lk(c->mu);
-        if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
-        if (bytes > c->host_cap) {
-            hipFree(c->host_d);
-            hipFree(c->host_dn);
-            c->host_d = c->host_dn = nullptr;
-            c->host_cap = 0;
-            if (hipMalloc(&c->host_d, bytes) != hipSuccess || hipMalloc(&c->host_dn, bytes) != hipSuccess) {
-                hipFree(c->host_d);
-                c->host_d = nullptr;
-                return c->set_error(GQS_ENOMEM, "gqs_sort: device buffers");
-            }
-            c->host_cap = bytes;
-        }
-        if (gqs_status s = c->host_copy((char*)c->host_d, (char*)d, bytes, true)) return s;
-    }
-    gqs_status s = gqs_sort_device(c, dtype, c->host_d, c->host_dn, n, nullptr);
-    if (s) return s;
+    // the whole upload / sort / download holds the lock: the device copies
+    // and the staging lanes belong to the context
     std::lock_guard<std::mutex> lk(c->mu);
-    // the copy lanes' streams do not wait for the sort's (null) stream
-    if (c->fail(hipStreamSynchronize(nullptr), "sort")) return GQS_EDEVICE;
+    if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+    if (bytes > c->host_cap) {
+        hipDeviceSynchronize();
+        hipFree(c->host_d);
+        hipFree(c->host_dn);
+        c->host_d = c->host_dn = nullptr;
+        c->host_cap = 0;
+        if (hipMalloc(&c->host_d, bytes) != hipSuccess || hipMalloc(&c->host_dn, bytes) != hipSuccess) {
+            hipFree(c->host_d);
+            c->host_d = nullptr;
+            return c->set_error(GQS_ENOMEM, "gqs_sort: device buffers");
+        }
+        c->host_cap = bytes;
+    }
+    // the copy lanes' streams do not wait for earlier sorts of this context
+    if (c->ws.done && c->fail(hipEventSynchronize(c->ws.done), "sync")) return GQS_EDEVICE;
+    if (gqs_status s = c->host_copy((char*)c->host_d, (char*)d, bytes, true)) return s;
+    if (gqs_status s = c->sort_device_locked(dtype, c->host_d, c->host_dn, n, nullptr)) return s;
+    // synchronous: the sticky status of this call decides the result
+    if (gqs_status s = c->fetch_stats()) return s;
+    if (gqs_status s = c->status_of(c->status_bits, "gqs_sort")) return s;
     return c->host_copy((char*)c->host_d, (char*)d, bytes, false);
+}
+
+namespace gqs {
+
+// One partition level of [0, n) of d into dn, without the local sort: the
+// level chain of run() for a single segment, stopped after the scatter.
+gqs_status Ctx::partition_level(int dtype, void* d, void* dn, uint64_t n, LevelResult* out) {
+    gqs_params p = run_params(dtype, n);
+    p.host_driven = 0;
+    if (gqs_status s = ensure_workspace(ws, n, 0, p, dtype)) return s;
+    hipStream_t st = nullptr;
+    if (gqs_status s = stream_enter(ws, st)) return s;
+    capturing_now = false;
+    if (fail(launch_init(ws.ctl, kCtlBlocks, false, ws.segs[0], 1, n, SEG_RAW, st), "init")) return GQS_EDEVICE;
+    LevelArgs a{};
+    a.d = d;
+    a.dn = dn;
+    a.segs = ws.segs[0];
+    a.next_segs = ws.segs[1];
+    a.plans = ws.plans;
+    a.spl = ws.spl;
+    a.meta = ws.meta;
+    a.lut = ws.lut;
+    a.mat = ws.mat;
+    a.part = ws.part;
+    a.bkt = ws.bkt;
+    a.fill = ws.fill;
+    a.ctl = &ws.ctl[0];
+    a.ctl_next = &ws.ctl[1];
+    a.mode = (uint32_t)p.mode;
+    a.leaf_max = p.leaf_max;
+    a.leaf_target = p.leaf_target;
+    a.oversample = p.oversample;
+    a.max_fanout = p.max_fanout;
+    a.is_last = 1;
+    a.seg_cap = (uint32_t)ws.cap.seg;
+    a.mat_cap = (uint32_t)ws.cap.mat;
+    a.bkt_cap = (uint32_t)ws.cap.bkt;
+    a.fill_cap = (uint32_t)ws.cap.fill;
+    a.seed = 0x5EEDu;
+    a.leaf_cap = leaf_cap(dtype);
+    a.huge = ws.huge;
+    a.huge_cap = (uint32_t)ws.cap.huge;
+    a.status = &ws.ctl[kSeedCtl].overflow;
+    const Grid g = grid(p);
+    for (int step = 0; step < kLevelSteps; ++step)
+        if (fail(launch_level_step(step, dtype, a, g, st), "level")) return GQS_EDEVICE;
+    if (gqs_status s = stream_leave(ws, st)) return s;
+    const size_t kb = key_bytes(dtype);
+    SegPlan plan{};
+    uint16_t meta[kSplStride];
+    if (fail(hipStreamSynchronize(st), "level sync") ||
+        fail(hipMemcpy(&plan, ws.plans, sizeof plan, hipMemcpyDeviceToHost), "plan readback") ||
+        fail(hipMemcpy(meta, ws.meta, sizeof meta, hipMemcpyDeviceToHost), "meta readback") ||
+        fail(hipMemcpy(out->splitters, ws.spl, kSplStride * kb, hipMemcpyDeviceToHost), "splitter readback"))
+        return GQS_EDEVICE;
+    const uint32_t nb = plan.nbuckets;
+    if (nb < 2 || nb > (uint32_t)kSplStride) return set_error(GQS_EDEVICE, "partition_level: bad bucket count");
+    // unique splitter i maps to bucket base[i] (+1 for its equality bucket);
+    // the first table entry past the splitters holds base nb - 1
+    uint32_t m = 0;
+    while (m < (uint32_t)kSplStride && (meta[m] & 0xFFu) != nb - 1) {
+        out->eq[m] = (uint8_t)((meta[m] >> 8) & 1u);
+        ++m;
+    }
+    out->nsplit = m;
+    out->nbuckets = nb;
+    std::vector<BucketRec> b(nb);
+    if (fail(hipMemcpy(b.data(), ws.bkt, nb * sizeof(BucketRec), hipMemcpyDeviceToHost), "bucket readback"))
+        return GQS_EDEVICE;
+    for (uint32_t i = 0; i < nb; ++i) {
+        out->start[i] = b[i].start;
+        out->size[i] = b[i].size & ~kBucketDirect;
+        out->bucket_eq[i] = (b[i].size & kBucketDirect) ? 1 : 0;
+    }
+    uint32_t bits = 0;
+    gqs_stats tmp{};
+    if (gqs_status s = fetch_stats(ws, &tmp, &bits)) return s;
+    return status_of(bits, "gqs_partition_level");
+}
+
+}  // namespace gqs
+
+gqs_status gqs_partition_level(gqs_ctx* c, int dtype, void* d, void* dn, size_t n, void* splitters, uint8_t* eq,
+                               uint32_t* nsplit, uint64_t* bucket_start, uint32_t* bucket_size, uint8_t* bucket_eq,
+                               uint32_t* nbuckets) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (dtype < DT_U32 || dtype > DT_F64) return c->set_error(GQS_EINVAL, "gqs_partition_level: bad dtype");
+    if (!d || !dn || !splitters || !eq || !nsplit || !bucket_start || !bucket_size || !bucket_eq || !nbuckets)
+        return c->set_error(GQS_EINVAL, "gqs_partition_level: null pointer");
+    if (n < 2 || n >= gqs::kMaxRunKeys) return c->set_error(GQS_EINVAL, "gqs_partition_level: n out of range");
+    if (c->fail(hipSetDevice(c->device), "hipSetDevice")) return GQS_EDEVICE;
+    const size_t kb = key_bytes(dtype);
+    std::vector<unsigned char> spl(gqs::kSplStride * kb);
+    gqs::LevelResult r{};
+    std::vector<uint64_t> st(gqs::kSplStride);
+    std::vector<uint32_t> sz(gqs::kSplStride);
+    std::vector<uint8_t> be(gqs::kSplStride);
+    r.splitters = spl.data();
+    r.eq = eq;
+    r.start = st.data();
+    r.size = sz.data();
+    r.bucket_eq = be.data();
+    gqs_status s = c->partition_level(dtype, d, dn, n, &r);
+    if (s != GQS_OK) return s;
+    std::memcpy(splitters, spl.data(), r.nsplit * kb);
+    *nsplit = r.nsplit;
+    *nbuckets = r.nbuckets;
+    std::memcpy(bucket_start, st.data(), r.nbuckets * sizeof(uint64_t));
+    std::memcpy(bucket_size, sz.data(), r.nbuckets * sizeof(uint32_t));
+    std::memcpy(bucket_eq, be.data(), r.nbuckets);
+    return GQS_OK;
 }
 
 gqs_status gqs_sort_each(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* host_offsets, size_t nseg,
@@ -698,7 +893,7 @@ gqs_status gqs_sort_each(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_
 // ---- captured segmented sort ------------------------------------------------
 struct gqs_graph {
     gqs_ctx* ctx = nullptr;
-    gqs::Workspace ws;
+    gqs::Workspace ws;  // ws.done / ws.last order replays on different streams
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -725,7 +920,7 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* c, int dtype, void* d, void* dn, 
     }
     hipStream_t cs = nullptr;
     if (c->fail(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream")) {
-        g->ws.release();
+        g->ws.release_all();
         delete g;
         return GQS_EDEVICE;
     }
@@ -746,7 +941,7 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* c, int dtype, void* d, void* dn, 
     if (st != GQS_OK) {
         if (g->exec) hipGraphExecDestroy(g->exec);
         if (g->graph) hipGraphDestroy(g->graph);
-        g->ws.release();
+        g->ws.release_all();
         delete g;
         return st;
     }
@@ -756,8 +951,20 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* c, int dtype, void* d, void* dn, 
 
 gqs_status gqs_graph_launch(gqs_graph* g, void* stream) {
     if (!g || !g->exec) return GQS_EINVAL;
-    if (g->ctx->fail(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch")) return GQS_EDEVICE;
-    return GQS_OK;
+    std::lock_guard<std::mutex> lk(g->ctx->mu);
+    hipStream_t st = (hipStream_t)stream;
+    if (gqs_status s = g->ctx->stream_enter(g->ws, st)) return s;
+    if (g->ctx->fail(hipGraphLaunch(g->exec, st), "hipGraphLaunch")) return GQS_EDEVICE;
+    return g->ctx->stream_leave(g->ws, st);
+}
+
+gqs_status gqs_graph_status(gqs_graph* g) {
+    if (!g) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(g->ctx->mu);
+    gqs_stats st{};
+    uint32_t bits = 0;
+    if (gqs_status s = g->ctx->fetch_stats(g->ws, &st, &bits)) return s;
+    return g->ctx->status_of(bits, "graph replay");
 }
 
 gqs_status gqs_graph_destroy(gqs_graph* g) {
@@ -765,7 +972,7 @@ gqs_status gqs_graph_destroy(gqs_graph* g) {
     hipDeviceSynchronize();
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
-    g->ws.release();
+    g->ws.release_all();
     delete g;
     return GQS_OK;
 }
@@ -775,7 +982,7 @@ gqs_status gqs_get_stats(gqs_ctx* c, gqs_stats* out) {
     std::lock_guard<std::mutex> lk(c->mu);
     if (gqs_status s = c->fetch_stats()) return s;
     *out = c->stats;
-    return GQS_OK;
+    return c->status_of(c->status_bits, "last sort");
 }
 
 gqs_status gqs_set_profiling(gqs_ctx* c, int on) {
@@ -792,7 +999,7 @@ gqs_status gqs_kernel_times(gqs_ctx* c, uint32_t* launches, double* ms, int nkin
         launches[k] = 0;
         ms[k] = 0.0;
     }
-    if (c->fail(hipStreamSynchronize(c->last_stream), "sync")) return GQS_EDEVICE;
+    if (c->ws.done && c->fail(hipEventSynchronize(c->ws.done), "sync")) return GQS_EDEVICE;
     c->trace.clear();
     for (auto& r : c->prof) {
         float t = 0.f;
@@ -819,6 +1026,33 @@ int gqs_kernel_trace(gqs_ctx* c, int* kinds, float* ms, int cap) {
         ms[i] = c->trace[i].second;
     }
     return n;
+}
+
+gqs_status gqs_device_report(gqs_ctx* c, gqs_device_props* out) {
+    if (!c || !out) return GQS_EINVAL;
+    hipDeviceProp_t p{};
+    if (c->fail(hipGetDeviceProperties(&p, c->device), "hipGetDeviceProperties")) return GQS_EDEVICE;
+    gqs_device_props r{};
+    std::snprintf(r.name, sizeof r.name, "%s", p.name);
+    std::snprintf(r.arch, sizeof r.arch, "%s", p.gcnArchName);
+    std::snprintf(r.pci_bus_id, sizeof r.pci_bus_id, "%04x:%02x:%02x.0", (unsigned)p.pciDomainID,
+                  (unsigned)p.pciBusID, (unsigned)p.pciDeviceID);
+    hipDriverGetVersion(&r.driver_version);
+    hipRuntimeGetVersion(&r.runtime_version);
+    r.compute_units = p.multiProcessorCount;
+    r.wave_size = p.warpSize;
+    r.max_threads_per_block = p.maxThreadsPerBlock;
+    for (int i = 0; i < 3; ++i) r.max_block_dim[i] = p.maxThreadsDim[i];
+    r.clock_khz = p.clockRate;
+    r.mem_clock_khz = p.memoryClockRate;
+    r.mem_bus_bits = p.memoryBusWidth;
+    r.lds_per_block = p.sharedMemPerBlock;
+    r.lds_per_cu = p.maxSharedMemoryPerMultiProcessor;
+    r.l2_bytes = (uint64_t)p.l2CacheSize;
+    r.global_mem_bytes = p.totalGlobalMem;
+    r.hbm_peak_gbs = 2.0 * (double)p.memoryClockRate * 1e3 * (double)p.memoryBusWidth / 8.0 / 1e9;
+    *out = r;
+    return GQS_OK;
 }
 
 gqs_status gqs_device_info(gqs_ctx* c, int* cus, int* lds_per_wg, int* wave_size, char* name, size_t name_len) {

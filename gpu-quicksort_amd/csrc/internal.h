@@ -37,7 +37,14 @@ struct Workspace {
     SegRec* huge = nullptr;  // large-leaf list (reused by every level)
     Caps cap;
     size_t key_bytes = 0;
-    void release();
+    // Stream order across calls: calls return once their kernels are
+    // enqueued, and every call on this workspace uses the same buffers, so a
+    // call on another stream first waits for `done`, recorded after the
+    // previous call's last launch on `last`.
+    hipStream_t last = nullptr;
+    hipEvent_t done = nullptr;
+    void release();      // device buffers (the stream-order event stays)
+    void release_all();  // buffers and event
 };
 
 struct ProfRec {
@@ -63,7 +70,36 @@ constexpr uint64_t kMaxRunKeys = (1ull << 32) - (1ull << 16);
 constexpr uint64_t kLargeRunKeys = 1ull << 31;
 
 Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
+// GQS_OK when every field of p is in the range the kernels assume.
+bool params_valid(const gqs_params& p);
 uint32_t levels_needed(uint64_t n, const gqs_params& p);
+
+// Control block of the arrays a sort starts from (the single range, or the
+// arrays of a segmented sort that go straight to a local sort), after the
+// kMaxLevels per-level blocks; its `overflow` word is the call's sticky
+// status, which every kernel that hits a bound also sets.
+constexpr int kSeedCtl = kMaxLevels;
+constexpr int kCtlBlocks = kMaxLevels + 1;
+
+// LevelCtl::overflow bits.
+constexpr uint32_t kOvfMatrix = 1u;     // count matrix or bucket table
+constexpr uint32_t kOvfSegs = 2u;       // next level's segment queue
+constexpr uint32_t kOvfFill = 4u;       // equality-bucket fill tasks
+constexpr uint32_t kOvfHuge = 8u;       // large-leaf list
+constexpr uint32_t kBadOffsets = 16u;   // an array's bounds are decreasing or beyond total_keys
+constexpr uint32_t kBadCount = 32u;     // more arrays than the graph was created for
+constexpr uint32_t kOvfCapacity = kOvfMatrix | kOvfSegs | kOvfFill | kOvfHuge;
+
+// Result of one partition level (host memory, kSplStride entries each).
+struct LevelResult {
+    void* splitters;     // unique splitters, ordered-key representation
+    uint8_t* eq;         // 1: the splitter has an equality bucket
+    uint32_t nsplit;
+    uint64_t* start;     // per bucket: first key
+    uint32_t* size;      // per bucket: keys
+    uint8_t* bucket_eq;  // 1: an equality bucket (its keys are not stored)
+    uint32_t nbuckets;
+};
 
 struct Ctx {
     int device = 0;
@@ -75,8 +111,9 @@ struct Ctx {
     std::mutex mu;
     std::string err;
     Workspace ws;
-    hipStream_t last_stream = nullptr;
     gqs_stats stats{};
+    uint32_t status_bits = 0;  // sticky overflow word of the last call (after fetch_stats)
+    uint32_t queue_limit = 0;  // testing hook: caps the segment queue (0: none)
     bool profiling = false;
     std::vector<ProfRec> prof;
     std::vector<std::pair<int, float>> trace;  // per launch of the last profiled sort
@@ -111,9 +148,20 @@ struct Ctx {
     void prof_end(hipStream_t st);
     // Enqueue one sort on `st` using workspace `w`.  nseg_dev (optional) is a
     // device word holding the number of arrays, read when the chain runs.
+    // keep_status: a later run of the same call (the large-n path) keeps the
+    // sticky status word that earlier runs may have set.
     gqs_status run(Workspace& w, int dtype, void* d, void* dn, uint64_t n, const uint64_t* offsets,
-                   uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing);
+                   uint32_t nseg_in, const uint32_t* nseg_dev, hipStream_t st, bool capturing,
+                   bool keep_status = false);
+    gqs_status partition_level(int dtype, void* d, void* dn, uint64_t n, LevelResult* out);
     gqs_status fetch_stats();
+    gqs_status fetch_stats(Workspace& w, gqs_stats* out, uint32_t* bits);
+    // Status of a call from its sticky overflow word.
+    gqs_status status_of(uint32_t bits, const char* what);
+    gqs_status stream_enter(Workspace& w, hipStream_t st);
+    gqs_status stream_leave(Workspace& w, hipStream_t st);
+    // gqs_sort_device without taking the lock (the caller holds it).
+    gqs_status sort_device_locked(int dtype, void* d, void* dn, uint64_t n, hipStream_t st);
     // Sorts beyond one run's 32-bit sizes: runs of `run_keys` keys sorted by
     // run(), then merged pairwise (merge scratch grown on demand).
     uint64_t large_run = 0;  // 0: default

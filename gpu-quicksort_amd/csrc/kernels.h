@@ -48,6 +48,7 @@ struct LevelArgs {
     uint32_t leaf_cap;       // largest leaf of one 512-thread work-group (keys)
     SegRec* huge;            // leaves above leaf_cap (and <= leaf_max), appended by k_plan
     uint32_t huge_cap;       // capacity of huge
+    uint32_t* status;        // the call's sticky overflow word (every bound hit is also ORed here)
 };
 
 // Local-sort (leaf) arguments.  Either an explicit list of segments
@@ -81,12 +82,24 @@ struct Grid {
 // flags: SEG_* of the single range, plus kInitLeaf (the range is a leaf of
 // the leaf list) or kInitHuge (a leaf of the large-leaf list).
 constexpr uint32_t kInitLeaf = 0x100u, kInitHuge = 0x200u;
-hipError_t launch_init(LevelCtl* ctl, int nlevels, SegRec* seg0, uint32_t seg0_count, uint64_t n,
+// Zeroes ctl[0, nblocks) except, with keep_status, the sticky status word
+// ctl[kSeedCtl].overflow (later runs of one call keep what earlier runs set).
+hipError_t launch_init(LevelCtl* ctl, int nblocks, bool keep_status, SegRec* seg0, uint32_t seg0_count, uint64_t n,
                        uint32_t flags, hipStream_t st);
 hipError_t launch_leaf(int dtype, const LeafArgs& a, const Grid& g, hipStream_t st);
-hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, SegRec* segs,
-                                LevelCtl* ctl0, SegRec* leaves, uint32_t* nleaves, uint32_t leaf_max,
-                                uint32_t leaf_cap, SegRec* huge, hipStream_t st);
+// Launchers of the key-type kernels (kernels_typed.hip, one instantiation per
+// key type): level steps 0 (splitters), 2 (count) and 6 (scatter), and the
+// local-sort launches of one pass.
+template <int DT> hipError_t typed_level_step(int step, const LevelArgs& a, const Grid& g, hipStream_t st);
+template <int DT> hipError_t typed_leaf(const LeafArgs& a, const Grid& g, hipStream_t st);
+// Arrays of a segmented sort: above leaf_max to the level-0 queue (counted in
+// ctl0->nseg), above leaf_cap to the large-leaf list and the rest to the leaf
+// list (both counted in seed->nhuge / seed->nleaf_list).  Arrays whose bounds
+// decrease or pass total_keys are skipped and flag kBadOffsets in
+// seed->overflow; a device count above nseg_in flags kBadCount.
+hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev,
+                                uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves,
+                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, hipStream_t st);
 
 // Merge of two sorted runs a[0, na) and b[0, nb) into out[0, na + nb) (out
 // overlaps neither run).  `split` holds merge_split_entries(dtype, na + nb)

@@ -213,22 +213,36 @@ gqs_status gqs_server_default_config(gqs_server_config* c) {
     return GQS_OK;
 }
 
+// Defaults and derived values of a server configuration.  A slot holds
+// slot_keys 8-byte keys, i.e. twice as many 4-byte keys; one segmented sort
+// takes fewer than 2^32 - 2^16 keys, so slot_keys is clamped to keep every
+// batch of 4-byte keys below that.
+gqs_status gqs_server_resolve_config(const gqs_server_config* in, gqs_server_config* out) {
+    if (!out) return GQS_EINVAL;
+    gqs_server_config c{};
+    gqs_server_default_config(&c);
+    if (in) c = *in;
+    if (c.slots < 1 || c.slots > 16 || c.max_batch_requests < 1 || c.hbm_budget_bytes == 0) return GQS_EINVAL;
+    // Budget: per slot, d and dn of slot_keys 8-byte keys plus the sort
+    // workspace (sized for slot_keys keys and as many arrays).
+    if (c.slot_keys == 0) c.slot_keys = c.hbm_budget_bytes / c.slots / 24;
+    c.slot_keys = std::min<uint64_t>(c.slot_keys, (gqs::kMaxRunKeys - 1) / 2);
+    if (c.slot_keys < 1) return GQS_EINVAL;
+    *out = c;
+    return GQS_OK;
+}
+
 gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_server** out) {
     if (!out) return GQS_EINVAL;
     *out = nullptr;
     gqs_server_config c{};
-    gqs_server_default_config(&c);
-    if (cfg) c = *cfg;
-    if (c.slots < 1 || c.slots > 16 || c.max_batch_requests < 1 || c.hbm_budget_bytes == 0) return GQS_EINVAL;
+    if (gqs_status st = gqs_server_resolve_config(cfg, &c)) return st;
     auto* s = new (std::nothrow) gqs_server();
     if (!s) return GQS_ENOMEM;
     if (gqs_status st = gqs_create(&s->ctx, device)) {
         delete s;
         return st;
     }
-    // Budget: per slot, d and dn of slot_keys 8-byte keys plus the sort
-    // workspace (sized for slot_keys keys and as many arrays).
-    if (c.slot_keys == 0) c.slot_keys = c.hbm_budget_bytes / c.slots / 24;
     s->cfg = c;
     s->slots.resize(c.slots);
     hipError_t e = hipSetDevice(device);
@@ -256,7 +270,7 @@ gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_serve
             hipFree(sl.dn);
             hipFree(sl.offs_dev);
             hipHostFree(sl.offs_host);
-            sl.ws.release();
+            sl.ws.release_all();
             if (sl.done) hipEventDestroy(sl.done);
             if (sl.stream) hipStreamDestroy(sl.stream);
         }
@@ -272,7 +286,8 @@ gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_serve
 gqs_status gqs_server_submit(gqs_server* s, int dtype, const void* in, void* out, size_t n, uint64_t* ticket) {
     if (!s || !ticket || (n && (!in || !out)) || dtype < GQS_U32 || dtype > GQS_F64) return GQS_EINVAL;
     const size_t kb = dtype == GQS_F64 ? 8 : 4;
-    if (n * kb > s->cfg.slot_keys * 8) return GQS_EBUSY;  // never fits a slot
+    // never fits a slot (slots hold fewer keys than one segmented sort's limit)
+    if (n * kb > s->cfg.slot_keys * 8 || n >= gqs::kMaxRunKeys) return GQS_EBUSY;
     auto* r = new (std::nothrow) Request();
     if (!r) return GQS_ENOMEM;
     r->dtype = dtype;
@@ -372,7 +387,7 @@ gqs_status gqs_server_destroy(gqs_server* s) {
         hipFree(sl.dn);
         hipFree(sl.offs_dev);
         hipHostFree(sl.offs_host);
-        sl.ws.release();
+        sl.ws.release_all();
         if (sl.done) hipEventDestroy(sl.done);
         if (sl.stream) hipStreamDestroy(sl.stream);
     }

@@ -65,6 +65,28 @@ class ServerStats(ctypes.Structure):
                                                "keys_sorted", "pending")]
 
 
+class DeviceProps(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 256), ("pci_bus_id", ctypes.c_char * 32),
+                ("driver_version", ctypes.c_int32), ("runtime_version", ctypes.c_int32),
+                ("compute_units", ctypes.c_int32), ("wave_size", ctypes.c_int32),
+                ("max_threads_per_block", ctypes.c_int32), ("max_block_dim", ctypes.c_int32 * 3),
+                ("clock_khz", ctypes.c_int32), ("mem_clock_khz", ctypes.c_int32), ("mem_bus_bits", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("lds_per_block", ctypes.c_uint64), ("lds_per_cu", ctypes.c_uint64),
+                ("l2_bytes", ctypes.c_uint64), ("global_mem_bytes", ctypes.c_uint64), ("hbm_peak_gbs", ctypes.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            if isinstance(v, bytes):
+                v = v.decode(errors="replace")
+            elif k == "max_block_dim":
+                v = list(v)
+            if k != "reserved":
+                out[k] = v
+        return out
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("levels_launched", ctypes.c_uint32), ("levels_used", ctypes.c_uint32),
                 ("oversize_leaves", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
@@ -80,6 +102,14 @@ _SIGS = {
     "gqs_get_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
     "gqs_set_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
     "gqs_default_params": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params)]),
+    "gqs_check_params": (ctypes.c_int, [ctypes.POINTER(Params)]),
+    "gqs_partition_level": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P, _P,
+                                           ctypes.POINTER(ctypes.c_uint32), _P, _P, _P,
+                                           ctypes.POINTER(ctypes.c_uint32)]),
+    "gqs_set_queue_limit": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "gqs_graph_status": (ctypes.c_int, [_P]),
+    "gqs_device_report": (ctypes.c_int, [_P, ctypes.POINTER(DeviceProps)]),
+    "gqs_server_resolve_config": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(ServerConfig)]),
     "gqs_sort": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     "gqs_sort_device": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
     "gqs_set_large_run": (ctypes.c_int, [_P, ctypes.c_size_t]),
@@ -167,6 +197,22 @@ def _stream(s):
     if hasattr(s, "cuda_stream"):
         return s.cuda_stream
     return int(s)
+
+
+def check_params(p):
+    """True when every field of p is in range (gqs_check_params; no device needed)."""
+    return load().gqs_check_params(ctypes.byref(p)) == 0
+
+
+def resolve_server_config(**kw):
+    """The configuration a server created with these fields runs with."""
+    cfg = ServerConfig()
+    _check(load().gqs_server_default_config(ctypes.byref(cfg)))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    out = ServerConfig()
+    _check(load().gqs_server_resolve_config(ctypes.byref(cfg), ctypes.byref(out)))
+    return out
 
 
 def default_params(dtype=U32):
@@ -289,6 +335,32 @@ class Context:
         self._lib.gqs_kernel_trace(self._h, kinds, ms, n)
         return [(KERNEL_KINDS[kinds[i]], ms[i]) for i in range(n)]
 
+    def partition_level(self, dtype, d, dn, n):
+        """One partition level of device keys d[0:n] into dn (gqs_partition_level).
+        Returns (splitters, eq flags, bucket starts, sizes, equality flags)."""
+        kd = np.uint64 if dtype == F64 else np.uint32
+        spl = np.zeros(256, kd)
+        eq = np.zeros(256, np.uint8)
+        ns, nb = ctypes.c_uint32(), ctypes.c_uint32()
+        st = np.zeros(256, np.uint64)
+        sz = np.zeros(256, np.uint32)
+        be = np.zeros(256, np.uint8)
+        _check(self._lib.gqs_partition_level(self._h, dtype, _ptr(d), _ptr(dn), n, spl.ctypes.data, eq.ctypes.data,
+                                             ctypes.byref(ns), st.ctypes.data, sz.ctypes.data, be.ctypes.data,
+                                             ctypes.byref(nb)), self)
+        m, b = ns.value, nb.value
+        return spl[:m], eq[:m], st[:b], sz[:b], be[:b]
+
+    def set_queue_limit(self, segments):
+        """Testing hook: cap the segment queue of later sorts (0: no cap)."""
+        _check(self._lib.gqs_set_queue_limit(self._h, segments), self)
+
+    def device_report(self):
+        """hipGetDeviceProperties report (the reference's show_CL output)."""
+        p = DeviceProps()
+        _check(self._lib.gqs_device_report(self._h, ctypes.byref(p)), self)
+        return p.as_dict()
+
     def device_info(self):
         cus, lds, wave = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         name = ctypes.create_string_buffer(128)
@@ -319,6 +391,23 @@ def batch_lengths(nseg, lo, hi, seed, total=None):
     return lens.astype(np.uint64)
 
 
+def request_stream(count, seed, lo_log2=10, hi_log2=26, mix=(0.6, 0.3, 0.1)):
+    """A serving workload: `count` requests with sizes floor(2^U), U uniform on
+    [lo_log2, hi_log2], and key types drawn 60% uint32 / 30% float / 10% double
+    (seeded).  Returns (sizes, dtypes, seeds); request i's keys are
+    generate(dtypes[i], "uniform" for uint32 else "normal", seeds[i], sizes[i])."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(2.0 ** rng.uniform(lo_log2, hi_log2, count)).astype(np.int64)
+    kinds = rng.choice([U32, F32, F64], size=count, p=list(mix)).astype(np.int64)
+    seeds = 5000 + 7919 * np.int64(seed) + np.arange(count, dtype=np.int64)
+    return sizes, kinds, seeds
+
+
+def request_keys(dtype, seed, n):
+    """Keys of one request of request_stream()."""
+    return generate(int(dtype), "uniform" if int(dtype) == U32 else "normal", int(seed), int(n))
+
+
 def generate(dtype, gen, seed, n):
     """Synthetic input of one of the benchmark generators (host numpy array)."""
     if isinstance(gen, str):
@@ -340,6 +429,10 @@ class SegmentedGraph:
 
     def launch(self, stream=None):
         _check(self.ctx._lib.gqs_graph_launch(self._h, _stream(stream)), self.ctx)
+
+    def status(self):
+        """Waits for the last replay; raises GQSError on bad offsets or overflow."""
+        _check(self.ctx._lib.gqs_graph_status(self._h), self.ctx)
 
     def close(self):
         if self._h:

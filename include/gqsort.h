@@ -19,9 +19,20 @@
  * GPU-Quicksort_Full_SYCL/QuicksortMain.cpp:1192-1194).  Float keys are
  * ordered as IEEE values for inputs without NaN; -0.0 sorts before +0.0.
  *
- * Threading: a context serialises its own calls with an internal lock; use one
- * context per host thread for concurrency.  Streams are passed as void*
- * (a hipStream_t; NULL is the default stream).
+ * Threading: a context serialises its own calls with an internal lock (a
+ * host-pointer gqs_sort holds it from upload to download); use one context
+ * per host thread for concurrency.  Streams are passed as void* (a
+ * hipStream_t; NULL is the default stream).  Stream-ordered calls on one
+ * context share its workspace: a call enqueued on another stream than the
+ * previous call first waits (hipStreamWaitEvent) for the previous call's work,
+ * so sorts on different streams of one context run one after the other.
+ *
+ * Status: the reference's failures are silent asserts (CL12/OpenCLUtils.h:
+ * 41-45).  Here a bound hit on the device (a queue capacity, or bad segment
+ * offsets) skips the affected keys, sets a sticky flag and is reported as
+ * GQS_EOVERFLOW (capacity) or GQS_EINVAL (offsets): synchronously by gqs_sort,
+ * and for the stream-ordered calls by the next gqs_get_stats on the context
+ * (gqs_graph_status for a captured graph), which waits for the call.
  */
 #ifndef GQSORT_H
 #define GQSORT_H
@@ -38,7 +49,7 @@ typedef enum gqs_status {
     GQS_EINVAL = 1,      /* null pointer, bad dtype, n out of range, bad parameter */
     GQS_ENOMEM = 2,      /* device or pinned-host allocation failed */
     GQS_EDEVICE = 3,     /* a HIP call failed (see gqs_last_error) */
-    GQS_EOVERFLOW = 4,   /* an internal queue bound was exceeded (should never happen) */
+    GQS_EOVERFLOW = 4,   /* an internal queue bound was exceeded; the affected keys are left unsorted */
     GQS_EBUSY = 5,       /* serving: memory budget or slots exhausted, request not admitted */
     GQS_ENOTFOUND = 6    /* serving: unknown ticket */
 } gqs_status;
@@ -88,6 +99,9 @@ const char* gqs_last_error(const gqs_ctx* ctx);
 gqs_status gqs_get_params(const gqs_ctx* ctx, gqs_params* out);
 gqs_status gqs_set_params(gqs_ctx* ctx, const gqs_params* p);
 gqs_status gqs_default_params(int dtype, gqs_params* out);
+/* GQS_OK when every field is in range (the checks of gqs_set_params and of
+ * every entry gqs_tune_load reads); needs no device. */
+gqs_status gqs_check_params(const gqs_params* p);
 
 /* Reference semantics: d, dn are HOST pointers; synchronous.  The keys are
  * copied to device buffers owned by the context, sorted and copied back. */
@@ -102,6 +116,23 @@ gqs_status gqs_sort(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n);
  * (each through the partition levels) merged pairwise with 64-bit indices,
  * ping-ponging between d and dn.  gqs_get_stats then describes the last run. */
 gqs_status gqs_sort_device(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n, void* stream);
+
+/* One partition level of the keys d[0, n) (host-synchronous), without the
+ * local sort: the reference's gqsort_kernel launch and the child records it
+ * returns (CL12/QuicksortMain.cpp:332-373, CL12/QuicksortKernels.cl:102-240),
+ * generalised to up to 255 splitters.  Keys are written to dn grouped by
+ * bucket, in ordered-key representation (uint32 unchanged; float and double
+ * bits mapped by the order-preserving bijection: non-negative values get the
+ * sign bit set, negative values all bits flipped).  Outputs, host memory:
+ * splitters[0, *nsplit) (strictly increasing, same representation, up to 255
+ * keys), eq[i] = 1 when splitter i has an equality bucket, and per bucket
+ * (up to 256) its first key, its size and whether it is an equality bucket.
+ * Bucket order: keys up to splitter 0, [keys equal to splitter 0], keys up to
+ * splitter 1, ...; an equality bucket's keys are not stored (the sort fills
+ * their value in afterwards, the reference's pivot fill, :214-216). */
+gqs_status gqs_partition_level(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n, void* splitters, uint8_t* eq,
+                               uint32_t* nsplit, uint64_t* bucket_start, uint32_t* bucket_size, uint8_t* bucket_eq,
+                               uint32_t* nbuckets);
 
 /* Run length of the large-n path: with keys > 0, every sort of more than
  * `keys` keys on this context takes that path with runs of `keys` keys
@@ -133,6 +164,10 @@ gqs_status gqs_graph_create_segmented(gqs_ctx* ctx, int dtype, void* d, void* dn
                                       const uint32_t* nseg_dev, size_t max_nseg, size_t max_total_keys,
                                       gqs_graph** out);
 gqs_status gqs_graph_launch(gqs_graph* g, void* stream);
+/* Waits for the last replay and returns its status: GQS_EINVAL if an array's
+ * offsets decreased or passed max_total_keys, or *nseg_dev exceeded max_nseg
+ * (those arrays were skipped), GQS_EOVERFLOW on a queue bound, else GQS_OK. */
+gqs_status gqs_graph_status(gqs_graph* g);
 gqs_status gqs_graph_destroy(gqs_graph* g);
 
 /* Autotuner (the reference hand-picks QUICKSORT_BLOCK_SIZE, SORT_THRESHOLD and
@@ -147,8 +182,15 @@ gqs_status gqs_autotune(gqs_ctx* ctx, int dtype, size_t n, int gen, uint32_t rep
 gqs_status gqs_tune_save(gqs_ctx* ctx, const char* path);
 gqs_status gqs_tune_load(gqs_ctx* ctx, const char* path);
 
-/* Statistics of the last sort (synchronises the context's last stream). */
+/* Statistics of the last sort (waits for it).  Returns the last call's status:
+ * GQS_EOVERFLOW if a queue bound was hit, GQS_EINVAL if segment offsets were
+ * out of range (out is filled either way). */
 gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
+
+/* Testing hook, no counterpart in the reference: caps the segment queue of
+ * every following sort on this context at `segments` entries (0 removes the
+ * cap), so that the overflow path can be driven from the host. */
+gqs_status gqs_set_queue_limit(gqs_ctx* ctx, uint32_t segments);
 
 /* Per-kernel timing with HIP events on the sort's own stream.  While enabled,
  * every launch is bracketed by events; gqs_kernel_times() synchronises and
@@ -164,6 +206,32 @@ int gqs_kernel_trace(gqs_ctx* ctx, int* kinds, float* ms, int cap);
 
 /* Device properties used for launch shapes (CUs, LDS per work-group, name). */
 gqs_status gqs_device_info(gqs_ctx* ctx, int* cus, int* lds_per_wg, int* wave_size, char* name, size_t name_len);
+
+/* The device report of the reference's show_CL option
+ * (QueryPrintOpenCLDeviceInfo, CL12/OpenCLUtils.h:291-387: vendor, name,
+ * driver version, compute units, work-item sizes, work-group size, clock,
+ * local and global memory), from hipGetDeviceProperties. */
+typedef struct gqs_device_props {
+    char name[256];             /* product name */
+    char arch[256];             /* ISA and target features, e.g. gfx950:sramecc+:xnack- */
+    char pci_bus_id[32];        /* domain:bus:device.function */
+    int32_t driver_version;     /* hipDriverGetVersion */
+    int32_t runtime_version;    /* hipRuntimeGetVersion */
+    int32_t compute_units;
+    int32_t wave_size;
+    int32_t max_threads_per_block;
+    int32_t max_block_dim[3];
+    int32_t clock_khz;          /* peak shader clock */
+    int32_t mem_clock_khz;      /* peak memory clock */
+    int32_t mem_bus_bits;       /* memory interface width */
+    int32_t reserved;
+    uint64_t lds_per_block;     /* bytes of LDS one work-group may use */
+    uint64_t lds_per_cu;        /* bytes of LDS per compute unit */
+    uint64_t l2_bytes;
+    uint64_t global_mem_bytes;  /* HBM capacity */
+    double hbm_peak_gbs;        /* 2 * mem_clock * mem_bus_bits / 8, GB/s */
+} gqs_device_props;
+gqs_status gqs_device_report(gqs_ctx* ctx, gqs_device_props* out);
 
 /* Synthetic inputs for the benchmark configurations, generated on the host
  * with a counter-based hash (deterministic for a seed, independent of thread
@@ -194,6 +262,10 @@ typedef struct gqs_server_stats {
     uint64_t submitted, completed, failed, batches, max_batch_requests, keys_sorted, pending;
 } gqs_server_stats;
 gqs_status gqs_server_default_config(gqs_server_config* cfg);
+/* The configuration a server created from `cfg` runs with (defaults filled,
+ * slot_keys derived from the budget when 0 and capped so that a batch of
+ * 4-byte keys stays below 2^32 - 2^16 keys); needs no device. */
+gqs_status gqs_server_resolve_config(const gqs_server_config* cfg, gqs_server_config* out);
 gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_server** out);
 /* GQS_EBUSY: the request can never fit a slot. */
 gqs_status gqs_server_submit(gqs_server* s, int dtype, const void* in, void* out, size_t n, uint64_t* ticket);

@@ -472,13 +472,35 @@ int orc_gpuqsort_f64(double* d, double* dn, size_t n, uint32_t qbs, uint32_t gq_
 
 }  // extern "C"
 
+// ------------------------------------------ arrays of ordered keys / segmented std::sort
+// orc_keys_*: the float -> ordered-key bijection (orc_f32_key / orc_f64_key)
+// over an array, for comparing the GPU's level output with its input.
+// orc_std_sort_segmented_*: std::sort of each array [off[i], off[i+1]) on
+// its own, the per-array oracle of a segmented sort.
+template <class T>
+static void std_sort_segmented(T* a, const uint64_t* off, size_t nseg) {
+    for (size_t i = 0; i < nseg; ++i) std::sort(a + off[i], a + off[i + 1]);
+}
+
+extern "C" {
+void orc_keys_f32(const uint32_t* in, uint32_t* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = orc_f32_key(in[i]);
+}
+void orc_keys_f64(const uint64_t* in, uint64_t* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = orc_f64_key(in[i]);
+}
+void orc_std_sort_segmented_u32(uint32_t* a, const uint64_t* off, size_t nseg) { std_sort_segmented(a, off, nseg); }
+void orc_std_sort_segmented_f32(float* a, const uint64_t* off, size_t nseg) { std_sort_segmented(a, off, nseg); }
+void orc_std_sort_segmented_f64(double* a, const uint64_t* off, size_t nseg) { std_sort_segmented(a, off, nseg); }
+}  // extern "C"
+
 // ------------------------------------------ k-way classification (brute force)
 // Splitters u[0..m-1] strictly increasing; eq[i] != 0 gives u[i] its own
 // equality bucket.  Bucket order: (-inf,u0) [=u0] (u0,u1) [=u1] ... (u_{m-1},+inf),
 // where a range bucket (u_{i-1}, u_i] also holds keys == u_i when eq[i] == 0.
 // Writes each key's bucket id and adds to counts[nbuckets].  Returns nbuckets.
 template <class K>
-static int kway_classify(const K* keys, size_t n, const K* u, const uint8_t* eq, int m, uint8_t* bucket_out,
+static int kway_classify(const K* keys, size_t n, const K* u, const uint8_t* eq, int m, uint16_t* bucket_out,
                          uint64_t* counts) {
     std::vector<int> id_range(m + 1), id_eq(m, -1);
     int next = 0;
@@ -494,7 +516,7 @@ static int kway_classify(const K* keys, size_t n, const K* u, const uint8_t* eq,
             else if (k == u[i]) b = eq[i] ? id_eq[i] : id_range[i];
         }
         if (b < 0) b = id_range[m];
-        if (bucket_out) bucket_out[j] = (uint8_t)b;
+        if (bucket_out) bucket_out[j] = (uint16_t)b;
         if (counts) counts[b]++;
     }
     return next;
@@ -502,11 +524,11 @@ static int kway_classify(const K* keys, size_t n, const K* u, const uint8_t* eq,
 
 extern "C" {
 int orc_kway_classify_u32(const uint32_t* keys, size_t n, const uint32_t* u, const uint8_t* eq, int m,
-                          uint8_t* bucket_out, uint64_t* counts) {
+                          uint16_t* bucket_out, uint64_t* counts) {
     return kway_classify(keys, n, u, eq, m, bucket_out, counts);
 }
 int orc_kway_classify_u64(const uint64_t* keys, size_t n, const uint64_t* u, const uint8_t* eq, int m,
-                          uint8_t* bucket_out, uint64_t* counts) {
+                          uint16_t* bucket_out, uint64_t* counts) {
     return kway_classify(keys, n, u, eq, m, bucket_out, counts);
 }
 

@@ -52,6 +52,10 @@ def load():
         lib.orc_f64_key.restype = ctypes.c_uint64
         lib.orc_f64_unkey.argtypes = [ctypes.c_uint64]
         lib.orc_f64_unkey.restype = ctypes.c_uint64
+        for t in ("u32", "f32", "f64"):
+            getattr(lib, f"orc_std_sort_segmented_{t}").argtypes = [P, P, ctypes.c_size_t]
+        lib.orc_keys_f32.argtypes = [P, P, ctypes.c_size_t]
+        lib.orc_keys_f64.argtypes = [P, P, ctypes.c_size_t]
         lib.orc_optp.argtypes = [ctypes.c_size_t, ctypes.c_double, ctypes.c_size_t]
         lib.orc_optp.restype = ctypes.c_size_t
         for t in ("u32", "u64"):
@@ -70,6 +74,30 @@ def std_sort(a):
     b = np.array(a, copy=True)
     getattr(load(), f"orc_std_sort_{_SUFFIX[b.dtype]}")(b.ctypes.data, b.size)
     return b
+
+
+def std_sort_segmented(a, offsets):
+    """std::sort of each array [offsets[i], offsets[i+1]) of a copy of a."""
+    b = np.array(a, copy=True)
+    o = np.ascontiguousarray(offsets, dtype=np.uint64)
+    getattr(load(), f"orc_std_sort_segmented_{_SUFFIX[b.dtype]}")(b.ctypes.data, o.ctypes.data, o.size - 1)
+    return b
+
+
+def ordered_keys(a):
+    """Keys in the sort's ordered-key representation: uint32 unchanged, float
+    and double bits through the order-preserving bijection."""
+    if a.dtype == np.uint32:
+        return np.array(a, copy=True)
+    if a.dtype == np.float32:
+        src = np.ascontiguousarray(a).view(np.uint32)
+        out = np.empty_like(src)
+        load().orc_keys_f32(src.ctypes.data, out.ctypes.data, src.size)
+        return out
+    src = np.ascontiguousarray(a).view(np.uint64)
+    out = np.empty_like(src)
+    load().orc_keys_f64(src.ctypes.data, out.ctypes.data, src.size)
+    return out
 
 
 def par_sort(a, threads):
@@ -103,7 +131,7 @@ def kway_classify(keys, splitters, eq):
     keys = np.ascontiguousarray(keys)
     u = np.ascontiguousarray(splitters, dtype=keys.dtype)
     e = np.ascontiguousarray(eq, dtype=np.uint8)
-    out = np.empty(keys.size, dtype=np.uint8)
+    out = np.empty(keys.size, dtype=np.uint16)
     counts = np.zeros(2 * len(u) + 1, dtype=np.uint64)
     t = "u64" if keys.dtype.itemsize == 8 else "u32"
     nb = getattr(load(), f"orc_kway_classify_{t}")(keys.ctypes.data, keys.size, u.ctypes.data, e.ctypes.data,

@@ -150,3 +150,42 @@ def test_cpp_dropin_compiles_and_links(tmp_path):
         assert r.stdout.strip() == "status 0 1 2 3"
     else:
         assert r.stdout.strip() == "status 3"
+
+
+# ------------------------------------------------------- parameter ranges
+def test_check_params_rejects_out_of_range_fields():
+    """The range checks shared by gqs_set_params and gqs_tune_load: values the
+    kernels index or divide by (a fan-out above the 256-entry splitter table,
+    leaf_target 0, a wave network above 512 keys, no work-groups) are refused."""
+    for dt in (gqsort.U32, gqsort.F32, gqsort.F64):
+        assert gqsort.check_params(gqsort.default_params(dt))
+    bad = [("max_fanout", 512), ("max_fanout", 3), ("max_fanout", 1), ("leaf_target", 0), ("oversample", 0),
+           ("bitonic_cutoff", 1024), ("bitonic_cutoff", 32), ("part_wgs_per_cu", 0), ("leaf_wgs_per_cu", 33),
+           ("leaf_max", 8), ("leaf_max", 32768), ("sort_threshold", 2), ("max_levels", 0), ("max_levels", 48),
+           ("mode", 2), ("host_driven", 2)]
+    for field, v in bad:
+        p = gqsort.default_params()
+        setattr(p, field, v)
+        assert not gqsort.check_params(p), (field, v)
+    p = gqsort.default_params()
+    p.max_fanout, p.leaf_max, p.max_levels = 2, 16, 47
+    assert gqsort.check_params(p)
+
+
+# ------------------------------------------------------- server config
+def test_server_config_caps_batches_below_one_run():
+    """A slot of slot_keys 8-byte keys holds twice as many 4-byte keys; one
+    segmented sort takes fewer than 2^32 - 2^16 keys, so the derived (or given)
+    slot size is capped to keep a batch of 4-byte keys below that."""
+    limit = (1 << 32) - (1 << 16)
+    c = gqsort.resolve_server_config(hbm_budget_bytes=64 << 30, slots=1)
+    assert c.slot_keys == (limit - 1) // 2 and 2 * c.slot_keys < limit
+    c = gqsort.resolve_server_config(slot_keys=1 << 40)
+    assert 2 * c.slot_keys < limit
+    c = gqsort.resolve_server_config(hbm_budget_bytes=1 << 30, slots=4)
+    assert c.slot_keys == (1 << 30) // 4 // 24
+    d = gqsort.resolve_server_config()
+    assert d.slots == 3 and d.hbm_budget_bytes == 64 << 30 and d.slot_keys == (64 << 30) // 3 // 24
+    for kw in ({"slots": 0}, {"slots": 17}, {"max_batch_requests": 0}, {"hbm_budget_bytes": 0}):
+        with pytest.raises(gqsort.GQSError):
+            gqsort.resolve_server_config(**kw)

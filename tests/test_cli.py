@@ -48,7 +48,14 @@ def test_reference_output_lines_all_types():
     print(r.stdout[-2000:], r.stderr[-500:], flush=True)
     assert r.returncode == 0, r.stderr
     out = r.stdout
-    assert "Device: " in out and "compute units: 256" in out
+    # the show_CL report: CUs, work-group sizes, clocks and memory (CL12/OpenCLUtils.h:291-387)
+    assert "HIP Device info:" in out and re.search(r"^MAX_COMPUTE_UNITS\s+:\s+256$", out, re.M)
+    assert re.search(r"^DEVICE_ARCH\s+:gfx950", out, re.M)
+    for key in ("MAX_CLOCK_FREQUENCY (MHz)", "MEMORY_CLOCK (MHz)", "MEMORY_BUS_WIDTH (bits)", "MAX_WORK_GROUP_SIZE"):
+        m = re.search(r"^" + re.escape(key) + r"\s*:\s*(\d+)$", out, re.M)
+        assert m and int(m.group(1)) > 0, key
+    m = re.search(r"^GLOBAL_MEM_SIZE\s+:\s*([0-9.]+)$", out, re.M)
+    assert m and float(m.group(1)) > 200e9  # 288 GB of HBM3E
     assert out.count("Allocating array size of 131072") == 3
     for t in ("uint", "float", "double"):
         assert f"Sorting with GPUQSort on the gpu with type {t}: " in out
@@ -67,6 +74,6 @@ def test_reference_default_uint_run():
     # the reference's own invocation shape: uint keys only, no device report
     r = run(1, "gpu", "amd", 1024, 1024, "no_show_CL")
     assert r.returncode == 0, r.stderr
-    assert "Device: " not in r.stdout
+    assert "HIP Device info:" not in r.stdout
     assert r.stdout.count("verifying: true") == 2
     assert " Number of failures: 0 out of 1" in r.stdout

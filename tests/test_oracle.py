@@ -181,3 +181,39 @@ def test_kway_invariants():
     # equality buckets hold exactly their splitter
     for b, v in ((1, 10), (4, 201), (6, 500)):
         assert np.all(a[ids == b] == v)
+
+
+def test_kway_bucket_ids_above_255():
+    """255 splitters, every one with an equality bucket: 511 buckets, ids
+    beyond one byte.  Closed form: id = 2 * #{u < k} + [k in u]."""
+    rng = np.random.default_rng(12)
+    u = np.unique(rng.integers(0, 1 << 31, 400, dtype=np.uint64).astype(np.uint32))[:255]
+    assert u.size == 255
+    a = np.concatenate([rng.integers(0, 1 << 31, 30000, dtype=np.uint64).astype(np.uint32), u, u[::3]])
+    ids, counts = orc.kway_classify(a, u, np.ones(255, np.uint8))
+    lt = np.searchsorted(u, a, side="left")
+    hit = (lt < u.size) & (u[np.minimum(lt, u.size - 1)] == a)
+    assert np.array_equal(ids.astype(np.int64), 2 * lt + hit)
+    assert len(counts) == 511 and int(counts.sum()) == a.size and ids.max() > 255
+
+
+def test_ordered_keys_and_segmented_sort():
+    """The array forms agree with the pinned scalar bijection (and so order like
+    the values), and the segmented std::sort equals numpy.sort per array."""
+    rng = np.random.default_rng(21)
+    for dt, ity, f in ((np.float32, np.uint32, "orc_f32_key"), (np.float64, np.uint64, "orc_f64_key")):
+        x = (rng.standard_normal(5000) * 1e3).astype(dt)
+        x[:4] = [0.0, np.inf, -np.inf, 1.0]
+        k = orc.ordered_keys(x)
+        scalar = getattr(orc.load(), f)
+        assert k.dtype == ity
+        assert all(int(k[i]) == scalar(int(b)) for i, b in enumerate(x.view(ity)[:300]))
+        assert np.array_equal(np.argsort(k, kind="stable"), np.argsort(x, kind="stable"))
+    u = rng.integers(0, 1 << 32, 1000, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(orc.ordered_keys(u), u)
+    for dt in (np.uint32, np.float32, np.float64):
+        a = (rng.integers(0, 1 << 31, 3000, dtype=np.uint64) if dt == np.uint32 else rng.standard_normal(3000)).astype(dt)
+        offs = np.array([0, 0, 1, 17, 17, 1500, 2999, 3000], np.uint64)
+        got = orc.std_sort_segmented(a, offs)
+        exp = np.concatenate([np.sort(a[int(offs[i]):int(offs[i + 1])]) for i in range(len(offs) - 1)])
+        assert np.array_equal(got, exp)
