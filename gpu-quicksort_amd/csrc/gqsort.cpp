@@ -287,8 +287,9 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         L = levels_needed(n, p);
         prof_begin(7, st);
         if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
+        const SeedCaps sc{(uint32_t)ws.cap.seg, (uint32_t)ws.cap.leaves, (uint32_t)ws.cap.huge};
         if (fail(launch_seed_segments(offsets, nseg_in, nseg_dev, n, ws.segs[0], &ws.ctl[0], &ws.ctl[kSeedCtl],
-                                      ws.leaves, p.leaf_max, leaf_cap(dtype), ws.huge, st),
+                                      ws.leaves, p.leaf_max, leaf_cap(dtype), ws.huge, sc, st),
                  "seed"))
             return GQS_EDEVICE;
         prof_end(st);
@@ -337,6 +338,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         la.dn = dn;
         la.list = ws.leaves;
         la.list_count = &ws.ctl[kSeedCtl].nleaf_list;
+        la.list_cap = (uint32_t)ws.cap.leaves;
         la.lv = a;
         la.lv.ctl = &ws.ctl[kSeedCtl];
         la.leaf_max = p.leaf_max;

@@ -70,6 +70,7 @@ constexpr uint64_t kMaxRunKeys = (1ull << 32) - (1ull << 16);
 constexpr uint64_t kLargeRunKeys = 1ull << 31;
 
 Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
+
 // GQS_OK when every field of p is in the range the kernels assume.
 bool params_valid(const gqs_params& p);
 uint32_t levels_needed(uint64_t n, const gqs_params& p);

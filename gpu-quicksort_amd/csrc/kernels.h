@@ -58,6 +58,7 @@ struct LeafArgs {
     void* dn;
     const SegRec* list;      // explicit leaves (raw or not), count in *list_count
     const uint32_t* list_count;
+    uint32_t list_cap;       // entries the list holds (a count above it was clamped and flagged)
     LevelArgs lv;            // bucket mode: the level whose buckets to sort
     uint32_t leaf_max;
     uint32_t is_last;
@@ -97,9 +98,12 @@ template <int DT> hipError_t typed_leaf(const LeafArgs& a, const Grid& g, hipStr
 // list (both counted in seed->nhuge / seed->nleaf_list).  Arrays whose bounds
 // decrease or pass total_keys are skipped and flag kBadOffsets in
 // seed->overflow; a device count above nseg_in flags kBadCount.
+struct SeedCaps {
+    uint32_t segs, leaves, huge;  // entries of the level-0 queue, the leaf list and the large-leaf list
+};
 hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev,
                                 uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves,
-                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, hipStream_t st);
+                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, SeedCaps caps, hipStream_t st);
 
 // Merge of two sorted runs a[0, na) and b[0, nb) into out[0, na + nb) (out
 // overlaps neither run).  `split` holds merge_split_entries(dtype, na + nb)

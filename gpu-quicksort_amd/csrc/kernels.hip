@@ -96,7 +96,7 @@ __global__ void k_init(LevelCtl* ctl, int nblocks, uint32_t keep_status, SegRec*
 // ctl0->nhuge by its plan) do not share a counter with them.
 __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, uint64_t total_keys,
                        SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves, uint32_t leaf_max,
-                       uint32_t leaf_cap, SegRec* huge) {
+                       uint32_t leaf_cap, SegRec* huge, SeedCaps caps) {
     uint32_t nseg = nseg_in;
     if (nseg_dev) {
         nseg = *nseg_dev;
@@ -113,10 +113,20 @@ __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t
         }
         const uint64_t len = hi - lo;
         if (len < 2) continue;
+        // Disjoint arrays within total_keys never exceed the capacities (they
+        // are sized from total_keys); the checks keep any other input in bounds.
         SegRec r{lo, (uint32_t)len, SEG_RAW};
-        if (len > leaf_max) segs[atomicAdd(&ctl0->nseg, 1u)] = r;
-        else if (len > leaf_cap) huge[atomicAdd(&seed->nhuge, 1u)] = r;
-        else leaves[atomicAdd(&seed->nleaf_list, 1u)] = r;
+        uint32_t slot, cap;
+        SegRec* list;
+        if (len > leaf_max) {
+            slot = atomicAdd(&ctl0->nseg, 1u), cap = caps.segs, list = segs;
+        } else if (len > leaf_cap) {
+            slot = atomicAdd(&seed->nhuge, 1u), cap = caps.huge, list = huge;
+        } else {
+            slot = atomicAdd(&seed->nleaf_list, 1u), cap = caps.leaves, list = leaves;
+        }
+        if (slot < cap) list[slot] = r;
+        else atomicOr(&seed->overflow, kOvfSegs);
     }
 }
 
@@ -124,7 +134,11 @@ __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t
 constexpr int kPlanWG = 1024;
 __global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
     __shared__ uint32_t wsum[kPlanWG / 64];
-    const uint32_t nseg = a.ctl->nseg;
+    // A count above the queue's capacity (children the previous level could
+    // not store) is clamped to the entries that exist, flagged, and written
+    // back below so that every later kernel of the level sees the clamped one.
+    const uint32_t nseg_raw = a.ctl->nseg;
+    const uint32_t nseg = min(nseg_raw, a.seg_cap);
     uint32_t ct = 0, cm = 0, cb = 0;
     uint64_t keys = 0;
     for (uint32_t base = 0; base < nseg; base += kPlanWG) {
@@ -155,6 +169,10 @@ __global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
     if (keys) atomicAdd(&skeys, (unsigned long long)keys);
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (nseg_raw > nseg) {
+            flag_overflow(a.ctl, a.status, kOvfSegs);
+            a.ctl->nseg = nseg;
+        }
         a.ctl->keys_lo = (uint32_t)skeys;
         a.ctl->keys_hi = (uint32_t)(skeys >> 32);
         if (cm > a.mat_cap || cb > a.bkt_cap) {
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
     __shared__ uint16_t eqi[kSplStride + 1];  // splitter index + 1 of an equality bucket, else 0
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t fbase;
-    const uint32_t nseg = a.ctl->nseg;
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const SegRec seg = a.segs[s];
         const SegPlan p = a.plans[s];
@@ -330,10 +348,10 @@ hipError_t launch_init(LevelCtl* ctl, int nblocks, bool keep_status, SegRec* seg
 
 hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev,
                                 uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves,
-                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, hipStream_t st) {
+                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, SeedCaps caps, hipStream_t st) {
     const uint32_t blocks = std::max(1u, std::min(1024u, (nseg_in + 255) / 256));
     hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(256), 0, st, offsets, nseg_in, nseg_dev, (unsigned long long)total_keys,
-                       segs, ctl0, seed, leaves, leaf_max, leaf_cap, huge);
+                       segs, ctl0, seed, leaves, leaf_max, leaf_cap, huge, caps);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ hipError_t typed_leaf(const LeafArgs& a, const Grid& g, hipStream_t st) {
         LeafArgs h = a;
         h.list = a.huge;
         h.list_count = a.huge_count;
+        h.list_cap = a.lv.huge_cap;
         hipLaunchKernelGGL((k_leaf_big<DT, true>), dim3(g.huge), dim3(2 * kLeafBigWG), 0, st, h);
     }
     return hipGetLastError();

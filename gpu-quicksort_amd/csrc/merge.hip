@@ -57,7 +57,12 @@ __global__ __launch_bounds__(kMergeWG) void k_merge_tile(const typename KeyOf<DT
     const uint32_t tid = threadIdx.x;
     const uint64_t g = blockIdx.x;
     const uint64_t p0 = g * T, p1 = min(p0 + T, na + nb);
-    const uint64_t a0 = split[g], a1 = split[g + 1];
+    // For sorted runs split[] is non-decreasing and the tile takes
+    // split[g+1] - split[g] <= T keys of A.  The clamp changes nothing then;
+    // for unsorted input (a run left unsorted by a reported overflow) it keeps
+    // every index within A, B and the tile.
+    const uint64_t a0 = split[g];
+    const uint64_t a1 = min(max(split[g + 1], a0), a0 + (p1 - p0));
     const uint64_t b0 = p0 - a0, b1 = p1 - a1;
     const uint32_t la = (uint32_t)(a1 - a0), lb = (uint32_t)(b1 - b0), len = la + lb;  // len <= T
     for (uint32_t q = tid; q < la; q += kMergeWG) s[q] = to_key<DT>(A[a0 + q]);

@@ -492,7 +492,7 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
     __shared__ uint32_t wsum[kSplWG / 64];
     __shared__ uint16_t lutw[kLutCells + 1];
     const uint32_t tid = threadIdx.x;
-    const uint32_t nseg = a.ctl->nseg;
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);  // clamped: see k_tileplan
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const SegRec seg = a.segs[s];
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(kPartWG) void k_count(LevelArgs a) {
     __shared__ TileLds<K> L;
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
-    const uint32_t nseg = a.ctl->nseg;
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);  // clamped: see k_tileplan
     uint32_t cur = 0xFFFFFFFFu;
     // Each work-group takes a contiguous run of tiles, so consecutive tiles
     // mostly share a segment (splitters stay in LDS) and thread 0 advances the
@@ -873,7 +873,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     __shared__ uint32_t mblk[kSplStride * kRow];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
-    const uint32_t nseg = a.ctl->nseg;
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);  // clamped: see k_tileplan
     const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
     uint32_t seg_i = 0, seg_next = 0;
@@ -1398,7 +1398,7 @@ __global__ __launch_bounds__(kLeafWG) void k_leaf_small(LeafArgs a) {
     __shared__ LeafDesc desc[4];
     const uint32_t tid = threadIdx.x, wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
-    const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
+    const uint32_t count = list_mode ? min(*a.list_count, a.list_cap) : a.lv.ctl->total_buckets;
     K* dout = (K*)a.d;
     for (uint32_t g = blockIdx.x; g * 4 < count; g += gridDim.x) {
         if (tid < 4) {
@@ -1503,7 +1503,7 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
     K* A = S.A;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
-    const uint32_t count = list_mode ? *a.list_count : a.lv.ctl->total_buckets;
+    const uint32_t count = list_mode ? min(*a.list_count, a.list_cap) : a.lv.ctl->total_buckets;
     const uint32_t G = gridDim.x;
     K* dout = (K*)a.d;
     auto big_only = [](LeafDesc ds) {
