@@ -182,7 +182,6 @@ void show_device(gqs_ctx* ctx) {
     printf("MAX_CLOCK_FREQUENCY (MHz)   :%8d\n", p.clock_khz / 1000);
     printf("MEMORY_CLOCK (MHz)          :%8d\n", p.mem_clock_khz / 1000);
     printf("MEMORY_BUS_WIDTH (bits)     :%8d\n", p.mem_bus_bits);
-    printf("PEAK_MEMORY_BANDWIDTH (GB/s):%12.1f\n", p.hbm_peak_gbs);
     printf("LOCAL_MEM_SIZE (per WG)     :%12.1f\n", (double)p.lds_per_block);
     printf("LOCAL_MEM_SIZE (per CU)     :%12.1f\n", (double)p.lds_per_cu);
     printf("L2_CACHE_SIZE               :%12.1f\n", (double)p.l2_bytes);

@@ -1035,7 +1035,7 @@ gqs_status gqs_device_report(gqs_ctx* c, gqs_device_props* out) {
     hipDeviceProp_t p{};
     if (c->fail(hipGetDeviceProperties(&p, c->device), "hipGetDeviceProperties")) return GQS_EDEVICE;
     gqs_device_props r{};
-    std::snprintf(r.name, sizeof r.name, "%s", p.name);
+    std::snprintf(r.name, sizeof r.name, "%s", p.name[0] ? p.name : p.gcnArchName);
     std::snprintf(r.arch, sizeof r.arch, "%s", p.gcnArchName);
     std::snprintf(r.pci_bus_id, sizeof r.pci_bus_id, "%04x:%02x:%02x.0", (unsigned)p.pciDomainID,
                   (unsigned)p.pciBusID, (unsigned)p.pciDeviceID);
@@ -1052,7 +1052,6 @@ gqs_status gqs_device_report(gqs_ctx* c, gqs_device_props* out) {
     r.lds_per_cu = p.maxSharedMemoryPerMultiProcessor;
     r.l2_bytes = (uint64_t)p.l2CacheSize;
     r.global_mem_bytes = p.totalGlobalMem;
-    r.hbm_peak_gbs = 2.0 * (double)p.memoryClockRate * 1e3 * (double)p.memoryBusWidth / 8.0 / 1e9;
     *out = r;
     return GQS_OK;
 }

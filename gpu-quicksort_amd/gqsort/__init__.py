@@ -72,7 +72,7 @@ class DeviceProps(ctypes.Structure):
                 ("max_threads_per_block", ctypes.c_int32), ("max_block_dim", ctypes.c_int32 * 3),
                 ("clock_khz", ctypes.c_int32), ("mem_clock_khz", ctypes.c_int32), ("mem_bus_bits", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("lds_per_block", ctypes.c_uint64), ("lds_per_cu", ctypes.c_uint64),
-                ("l2_bytes", ctypes.c_uint64), ("global_mem_bytes", ctypes.c_uint64), ("hbm_peak_gbs", ctypes.c_double)]
+                ("l2_bytes", ctypes.c_uint64), ("global_mem_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         out = {}

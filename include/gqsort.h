@@ -212,7 +212,7 @@ gqs_status gqs_device_info(gqs_ctx* ctx, int* cus, int* lds_per_wg, int* wave_si
  * driver version, compute units, work-item sizes, work-group size, clock,
  * local and global memory), from hipGetDeviceProperties. */
 typedef struct gqs_device_props {
-    char name[256];             /* product name */
+    char name[256];             /* product name (the ISA name when the driver reports none) */
     char arch[256];             /* ISA and target features, e.g. gfx950:sramecc+:xnack- */
     char pci_bus_id[32];        /* domain:bus:device.function */
     int32_t driver_version;     /* hipDriverGetVersion */
@@ -229,7 +229,6 @@ typedef struct gqs_device_props {
     uint64_t lds_per_cu;        /* bytes of LDS per compute unit */
     uint64_t l2_bytes;
     uint64_t global_mem_bytes;  /* HBM capacity */
-    double hbm_peak_gbs;        /* 2 * mem_clock * mem_bus_bits / 8, GB/s */
 } gqs_device_props;
 gqs_status gqs_device_report(gqs_ctx* ctx, gqs_device_props* out);
 

@@ -54,6 +54,7 @@ extern "C" {
 void orc_std_sort_u32(uint32_t* a, size_t n) { std::sort(a, a + n); }
 void orc_std_sort_f32(float* a, size_t n) { std::sort(a, a + n); }
 void orc_std_sort_f64(double* a, size_t n) { std::sort(a, a + n); }
+void orc_std_sort_u64(uint64_t* a, size_t n) { std::sort(a, a + n); }
 
 // Bitwise mismatch count between two arrays of `bytes_per_key`-wide keys.
 // Returns the number of differing keys; *first receives the first index

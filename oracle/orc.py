@@ -40,6 +40,7 @@ def load():
             f = getattr(lib, f"orc_gpuqsort_{t}")
             f.argtypes = [P, P, ctypes.c_size_t] + [ctypes.c_uint32] * 4 + [ctypes.POINTER(GpuqsortStats)]
             f.restype = ctypes.c_int
+        lib.orc_std_sort_u64.argtypes = [P, ctypes.c_size_t]
         lib.orc_mismatch.argtypes = [P, P, ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         lib.orc_mismatch.restype = ctypes.c_size_t
         for f in ("orc_median_u32", "orc_host_median_u32", "orc_f32_key", "orc_f32_unkey"):
@@ -66,7 +67,8 @@ def load():
     return _lib
 
 
-_SUFFIX = {np.dtype(np.uint32): "u32", np.dtype(np.float32): "f32", np.dtype(np.float64): "f64"}
+_SUFFIX = {np.dtype(np.uint32): "u32", np.dtype(np.float32): "f32", np.dtype(np.float64): "f64",
+           np.dtype(np.uint64): "u64"}
 
 
 def std_sort(a):

@@ -217,3 +217,10 @@ def test_ordered_keys_and_segmented_sort():
         got = orc.std_sort_segmented(a, offs)
         exp = np.concatenate([np.sort(a[int(offs[i]):int(offs[i + 1])]) for i in range(len(offs) - 1)])
         assert np.array_equal(got, exp)
+
+
+def test_std_sort_u64_matches_numpy():
+    rng = np.random.default_rng(31)
+    a = rng.integers(0, np.iinfo(np.uint64).max, 100001, dtype=np.uint64, endpoint=True)
+    a[::7] = a[3]
+    assert np.array_equal(orc.std_sort(a), np.sort(a))
