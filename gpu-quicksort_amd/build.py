@@ -18,7 +18,7 @@ ARCH = os.environ.get("GQS_OFFLOAD_ARCH", "gfx950")
 # (source, extra defines): kernels_typed.hip is compiled once per key type.
 UNITS = [("kernels.hip", []), ("kernels_typed.hip", ["-DGQS_DT=0"]), ("kernels_typed.hip", ["-DGQS_DT=1"]),
          ("kernels_typed.hip", ["-DGQS_DT=2"]), ("merge.hip", []), ("gqsort.cpp", []), ("devutil.cpp", []),
-         ("generate.cpp", []), ("serve.cpp", []), ("autotune.cpp", [])]
+         ("generate.cpp", []), ("serve.cpp", []), ("autotune.cpp", []), ("noalloc.hip", [])]
 OBJ_DIR = os.path.join(HERE, "_build")
 
 # Packed FP32 instructions are disabled for device code; the host pass of

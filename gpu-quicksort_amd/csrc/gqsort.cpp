@@ -472,6 +472,9 @@ gqs_status gqs_create(gqs_ctx** out, int device) {
         delete c;
         return GQS_EDEVICE;
     }
+    // the allocation-free entry point's scratch, looked up once per device
+    // with the device current (a failure only disables that entry point)
+    gqs::noalloc_setup(device, &c->noalloc_scratch);
     c->cus = prop.multiProcessorCount;
     c->lds_per_wg = (int)prop.sharedMemPerBlock;
     c->wave = prop.warpSize;

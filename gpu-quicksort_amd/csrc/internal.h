@@ -71,6 +71,13 @@ constexpr uint64_t kLargeRunKeys = 1ull << 31;
 
 Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype);
 
+// Allocation-free sorts (gqs_sort_device_noalloc): at most kNoAllocMaxKeys
+// keys, workspace carved out of a kNoAllocScratchBytes device array per GPU.
+constexpr uint64_t kNoAllocMaxKeys = 1ull << 24;
+constexpr size_t kNoAllocScratchBytes = 24ull << 20;
+// Device address of that array on `device` (the current device), looked up
+// once per device.
+gqs_status noalloc_setup(int device, unsigned char** out);
 // GQS_OK when every field of p is in the range the kernels assume.
 bool params_valid(const gqs_params& p);
 uint32_t levels_needed(uint64_t n, const gqs_params& p);
@@ -115,6 +122,7 @@ struct Ctx {
     gqs_stats stats{};
     uint32_t status_bits = 0;  // sticky overflow word of the last call (after fetch_stats)
     uint32_t queue_limit = 0;  // testing hook: caps the segment queue (0: none)
+    unsigned char* noalloc_scratch = nullptr;  // this device's allocation-free workspace (noalloc.hip)
     bool profiling = false;
     std::vector<ProfRec> prof;
     std::vector<std::pair<int, float>> trace;  // per launch of the last profiled sort

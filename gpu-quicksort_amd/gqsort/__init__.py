@@ -113,6 +113,7 @@ _SIGS = {
     "gqs_sort": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     "gqs_sort_device": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
     "gqs_set_large_run": (ctypes.c_int, [_P, ctypes.c_size_t]),
+    "gqs_sort_device_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
     "gqs_sort_segmented": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t, _P]),
     "gqs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     "gqs_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),

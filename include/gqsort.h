@@ -134,6 +134,18 @@ gqs_status gqs_partition_level(gqs_ctx* ctx, int dtype, void* d, void* dn, size_
                                uint32_t* nsplit, uint64_t* bucket_start, uint32_t* bucket_size, uint8_t* bucket_eq,
                                uint32_t* nbuckets);
 
+/* No counterpart in the reference: gqs_sort_device without any allocation,
+ * for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The caller passes d
+ * and dn as for gqs_sort_device; the sort's workspace comes from a 24 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends.  Limits: n <= 2^24 keys (GQS_EINVAL above, or when the current
+ * parameters need a larger workspace); no per-launch profiling; the large-n
+ * run length (gqs_set_large_run) does not apply.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (from any context, stream or
+ * thread) race, so the caller orders them (one stream, or waits between). */
+gqs_status gqs_sort_device_noalloc(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n, void* stream);
+
 /* Run length of the large-n path: with keys > 0, every sort of more than
  * `keys` keys on this context takes that path with runs of `keys` keys
  * (2 <= keys < 2^32 - 2^16); keys == 0 restores the default (runs of 2^31
