@@ -34,7 +34,7 @@ DTYPES = {"u32": gqsort.U32, "f32": gqsort.F32, "f64": gqsort.F64}
 # Algorithmic HBM bytes per key processed, per kernel kind (key bytes kb):
 # count reads each key once, scatter and the local sort read and write it
 # once (equality buckets are filled inside the local-sort launches).
-ALG_BYTES_PER_KEY = {"count": 1, "scatter": 2, "leaf": 2}
+ALG_BYTES_PER_KEY = {"count": 1, "scatter": 2, "leaf": 2, "tilepart": 2}
 
 
 def log(msg):
@@ -42,7 +42,8 @@ def log(msg):
 
 
 # kernels behind each kind of the library's per-kind timing
-KIND_KERNELS = {"count": ("k_count",), "scatter": ("k_scatter",), "leaf": ("k_leaf_big", "k_leaf_small")}
+KIND_KERNELS = {"count": ("k_count",), "scatter": ("k_scatter",), "leaf": ("k_leaf_big", "k_leaf_small"),
+                "tilepart": ("k_tilepart",)}
 
 
 def measured_traffic(kind, workload):
@@ -70,9 +71,13 @@ def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb, workload):
     ctx.set_profiling(False)
     st = ctx.stats()
     level_keys = [int(k) for k in st.level_keys]
-    keys_partitioned = sum(level_keys)
+    # the level before the spare one partitions tile-locally (k_tilepart),
+    # the others count and scatter
+    local = st.levels_launched - 2 if st.levels_launched >= 2 else None
+    local_keys = level_keys[local] if local is not None and local < len(level_keys) else 0
+    keys_partitioned = sum(level_keys) - local_keys
     # keys handled per kind over the whole sort
-    keys = {"count": keys_partitioned, "scatter": keys_partitioned, "leaf": n}
+    keys = {"count": keys_partitioned, "scatter": keys_partitioned, "leaf": n, "tilepart": local_keys}
     dom = max((k for k in ALG_BYTES_PER_KEY if times[k][0]), key=lambda k: times[k][1])
     launches, ms = times[dom]
     alg_bytes_per_launch = ALG_BYTES_PER_KEY[dom] * kb * keys[dom] / launches

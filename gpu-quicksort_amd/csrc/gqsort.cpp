@@ -64,6 +64,7 @@ void Workspace::release() {
     hipFree(meta);
     hipFree(lut);
     hipFree(mat);
+    hipFree(bsum);
     hipFree(part);
     hipFree(bkt);
     hipFree(fill);
@@ -99,8 +100,9 @@ gqs_status Ctx::stream_leave(Workspace& w, hipStream_t st) {
 // Capacities from provable bounds (n = total keys, S = segments per level):
 //   every segment of a level holds > leaf_max keys  =>  S <= n/(leaf_max+1)
 //   multipivot fan-out k <= max(2, 2*ceil(n_s/leaf_target)), buckets <= 2k-1
-//   count matrix <= sum_s ntiles_s * nbuckets_s
-//               <= 256*(n/tile) + sum_s nbuckets_s
+//   count matrix <= sum_s ntiles_s * (nbuckets_s + 1)   (+1: a tile-local
+//                   level's row of tile sizes)
+//               <= 257*(n/tile) + sum_s nbuckets_s + S
 Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     Caps c;
     const uint64_t tile = (dtype == DT_F64) ? TileShape<uint64_t>::kTileKeys : TileShape<uint32_t>::kTileKeys;
@@ -108,7 +110,7 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     const uint64_t lt = std::max<uint64_t>(1, p.leaf_target);
     const uint64_t sum_nb = (p.mode == MODE_CLASSIC) ? 3 * c.seg : 8 * c.seg + 4 * (n / lt + 1);
     c.bkt = sum_nb + 256;
-    c.mat = 256 * (n / tile + 1) + sum_nb + 256;
+    c.mat = 257 * (n / tile + 1) + sum_nb + c.seg + 256;
     c.part = c.mat / kScanChunk + 2;
     c.leaves = std::max<uint64_t>(1, nseg_in);
     // large leaves per level (or per segmented batch): each above one
@@ -192,6 +194,7 @@ gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
     if (!e) e = dalloc(&ws.meta, c.seg * kSplStride);
     if (!e) e = dalloc(&ws.lut, c.seg * kLutCells);
     if (!e) e = dalloc(&ws.mat, c.mat);
+    if (!e) e = dalloc(&ws.bsum, c.seg * kSplStride);
     if (!e) e = dalloc(&ws.part, c.part);
     if (!e) e = dalloc(&ws.bkt, c.bkt);
     if (!e) e = dalloc(&ws.fill, c.fill);
@@ -236,6 +239,7 @@ Grid Ctx::grid(const gqs_params& p) const {
     g.plan = cus * 2;
     g.leaf = cus * (int)p.leaf_wgs_per_cu;
     g.huge = cus;
+    g.tilepart = cus;
     return g;
 }
 
@@ -286,7 +290,8 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     if (offsets) {
         L = levels_needed(n, p);
         prof_begin(7, st);
-        if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], 0, 0, 0, st), "init")) return GQS_EDEVICE;
+        if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], ws.leaves, 0, 0, 0, st), "init"))
+            return GQS_EDEVICE;
         const SeedCaps sc{(uint32_t)ws.cap.seg, (uint32_t)ws.cap.leaves, (uint32_t)ws.cap.huge};
         if (fail(launch_seed_segments(offsets, nseg_in, nseg_dev, n, ws.segs[0], &ws.ctl[0], &ws.ctl[kSeedCtl],
                                       ws.leaves, p.leaf_max, leaf_cap(dtype), ws.huge, sc, st),
@@ -298,8 +303,8 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         const bool as_huge = as_leaf && n > leaf_cap(dtype);
         L = as_leaf ? 0 : levels_needed(n, p);
         prof_begin(7, st);
-        if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, as_huge ? ws.huge : (as_leaf ? ws.leaves : ws.segs[0]),
-                             1, n, SEG_RAW | (as_huge ? kInitHuge : (as_leaf ? kInitLeaf : 0u)), st),
+        if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], as_huge ? ws.huge : ws.leaves, 1, n,
+                             SEG_RAW | (as_huge ? kInitHuge : (as_leaf ? kInitLeaf : 0u)), st),
                  "init"))
             return GQS_EDEVICE;
         prof_end(st);
@@ -312,6 +317,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     a.meta = ws.meta;
     a.lut = ws.lut;
     a.mat = ws.mat;
+    a.bsum = ws.bsum;
     a.part = ws.part;
     a.bkt = ws.bkt;
     a.fill = ws.fill;
@@ -361,7 +367,14 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         a.ctl_next = &ws.ctl[lvl + 1];
         a.is_last = (lvl + 1 == L) ? 1u : 0u;
         a.seed = 0x5EEDu + 977u * lvl;
-        for (int step = 0; step < kLevelSteps; ++step) {
+        // The level planned to leave only leaves (the one before the spare)
+        // partitions tile-locally; the spare level, contiguous, takes any
+        // bucket it left above leaf_max.
+        a.tile_local = (p.mode == MODE_MULTIPIVOT && lvl + 2 == L) ? 1u : 0u;
+        const int* steps = a.tile_local ? kLocalSteps : nullptr;
+        const int nsteps = a.tile_local ? 4 : kLevelSteps;
+        for (int k = 0; k < nsteps; ++k) {
+            const int step = steps ? steps[k] : k;
             prof_begin(step_kind(step), st);
             if (fail(launch_level_step(step, dtype, a, g, st), "level")) return GQS_EDEVICE;
             prof_end(st);
@@ -779,7 +792,8 @@ gqs_status Ctx::partition_level(int dtype, void* d, void* dn, uint64_t n, LevelR
     hipStream_t st = nullptr;
     if (gqs_status s = stream_enter(ws, st)) return s;
     capturing_now = false;
-    if (fail(launch_init(ws.ctl, kCtlBlocks, false, ws.segs[0], 1, n, SEG_RAW, st), "init")) return GQS_EDEVICE;
+    if (fail(launch_init(ws.ctl, kCtlBlocks, false, ws.segs[0], ws.leaves, 1, n, SEG_RAW, st), "init"))
+        return GQS_EDEVICE;
     LevelArgs a{};
     a.d = d;
     a.dn = dn;
@@ -790,6 +804,7 @@ gqs_status Ctx::partition_level(int dtype, void* d, void* dn, uint64_t n, LevelR
     a.meta = ws.meta;
     a.lut = ws.lut;
     a.mat = ws.mat;
+    a.bsum = ws.bsum;
     a.part = ws.part;
     a.bkt = ws.bkt;
     a.fill = ws.fill;
@@ -833,13 +848,13 @@ gqs_status Ctx::partition_level(int dtype, void* d, void* dn, uint64_t n, LevelR
     }
     out->nsplit = m;
     out->nbuckets = nb;
-    std::vector<BucketRec> b(nb);
-    if (fail(hipMemcpy(b.data(), ws.bkt, nb * sizeof(BucketRec), hipMemcpyDeviceToHost), "bucket readback"))
+    std::vector<LeafRec> b(nb);
+    if (fail(hipMemcpy(b.data(), ws.bkt, nb * sizeof(LeafRec), hipMemcpyDeviceToHost), "bucket readback"))
         return GQS_EDEVICE;
     for (uint32_t i = 0; i < nb; ++i) {
         out->start[i] = b[i].start;
-        out->size[i] = b[i].size & ~kBucketDirect;
-        out->bucket_eq[i] = (b[i].size & kBucketDirect) ? 1 : 0;
+        out->size[i] = b[i].n & ~kBucketDirect;
+        out->bucket_eq[i] = (b[i].n & kBucketDirect) ? 1 : 0;
     }
     uint32_t bits = 0;
     gqs_stats tmp{};

@@ -29,12 +29,13 @@ struct Workspace {
     uint16_t* meta = nullptr;
     uint8_t* lut = nullptr;
     uint32_t* mat = nullptr;
+    uint32_t* bsum = nullptr;  // keys per bucket of a tile-local level (kSplStride per segment)
     uint32_t* part = nullptr;
-    BucketRec* bkt = nullptr;
+    LeafRec* bkt = nullptr;
     FillTask* fill = nullptr;
     LevelCtl* ctl = nullptr;
-    SegRec* leaves = nullptr;
-    SegRec* huge = nullptr;  // large-leaf list (reused by every level)
+    LeafRec* leaves = nullptr;
+    LeafRec* huge = nullptr;  // large-leaf list (reused by every level)
     Caps cap;
     size_t key_bytes = 0;
     // Stream order across calls: calls return once their kernels are
@@ -52,12 +53,15 @@ struct ProfRec {
     hipEvent_t a = nullptr, b = nullptr;
 };
 
-// Steps of one partition level, in launch order, and their profiling kinds.
+// Steps of one partition level and their profiling kinds.  Step ids: 0
+// splitters, 1 tileplan, 2 count, 3 scan_reduce, 4 scan_apply, 5 plan, 6
+// scatter, 7 tilepart, 8 plan of a tile-local level.  A contiguous level runs
+// 0-6, a tile-local level 0, 1, 7, 8 (equality buckets are filled by the
+// level's leaf pass either way).
 constexpr int kLevelSteps = 7;
+constexpr int kLocalSteps[4] = {0, 1, 7, 8};
 inline int step_kind(int step) {
-    // splitters, tileplan, count, scan_reduce, scan_apply, plan, scatter
-    // (equality buckets are filled by the level's leaf pass)
-    static const int kinds[kLevelSteps] = {0, 1, 2, 3, 3, 4, 5};
+    static const int kinds[9] = {0, 1, 2, 3, 3, 4, 5, 9, 4};
     return kinds[step];
 }
 hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st);

@@ -7,15 +7,10 @@
 
 namespace gqs {
 
-// Per-bucket result of one partition level: where the bucket starts inside
-// its segment and how many keys it holds.  Bit 31 of `size` marks an
+// Per-bucket result of one partition level (a LeafRec in the level's bucket
+// table): where the bucket's keys are and how many.  Bit 31 of `n` marks an
 // equality bucket: its keys all equal one splitter, the scatter does not store
 // them and the level's leaf pass writes the value into d after the scatter.
-struct BucketRec {
-    uint64_t start;   // absolute index of the bucket's first key
-    uint32_t size;    // keys (bit 31: equality bucket)
-    uint32_t in_dn;   // 1 when the bucket's keys were written to dn
-};
 constexpr uint32_t kBucketDirect = 0x80000000u;
 
 // Everything one partition level reads and writes.  Device pointers only.
@@ -29,8 +24,9 @@ struct LevelArgs {
     uint16_t* meta;          // bucket-id table, kSplStride per segment
     uint8_t* lut;            // splitter lookup table, kLutCells per segment
     uint32_t* mat;           // count matrix (scanned in place)
+    uint32_t* bsum;          // tile-local level: keys per bucket, kSplStride per segment (k_tilepart adds)
     uint32_t* part;          // scan chunk partials
-    BucketRec* bkt;          // per bucket of this level
+    LeafRec* bkt;            // per bucket of this level
     FillTask* fill;          // equality-bucket fill tasks of this level
     LevelCtl* ctl;           // this level
     LevelCtl* ctl_next;      // next level
@@ -46,8 +42,10 @@ struct LevelArgs {
     uint32_t fill_cap;       // capacity of fill
     uint32_t seed;           // sample-position hash seed
     uint32_t leaf_cap;       // largest leaf of one 512-thread work-group (keys)
-    SegRec* huge;            // leaves above leaf_cap (and <= leaf_max), appended by k_plan
+    LeafRec* huge;           // leaves above leaf_cap (and <= leaf_max), appended by k_plan
     uint32_t huge_cap;       // capacity of huge
+    uint32_t tile_local;     // 1: this level partitions each tile in place into dn (k_tilepart);
+                             // its buckets are gathered leaves, those above leaf_max rescue segments
     uint32_t* status;        // the call's sticky overflow word (every bound hit is also ORed here)
 };
 
@@ -56,7 +54,7 @@ struct LevelArgs {
 struct LeafArgs {
     void* d;
     void* dn;
-    const SegRec* list;      // explicit leaves (raw or not), count in *list_count
+    const LeafRec* list;     // explicit leaves (raw or not), count in *list_count
     const uint32_t* list_count;
     uint32_t list_cap;       // entries the list holds (a count above it was clamped and flagged)
     LevelArgs lv;            // bucket mode: the level whose buckets to sort
@@ -66,7 +64,7 @@ struct LeafArgs {
     uint32_t bitonic_cutoff;   // largest sub-bucket one wave sorts (<= 512)
     uint32_t leaf_cap;         // largest leaf of one 512-thread work-group; larger
                                // ones (<= leaf_max) come from the large-leaf list
-    const SegRec* huge;        // the large-leaf list of this pass, count in *huge_count
+    const LeafRec* huge;       // the large-leaf list of this pass, count in *huge_count
     const uint32_t* huge_count;
 };
 
@@ -77,6 +75,7 @@ struct Grid {
     int plan;       // work-groups of the plan kernel
     int leaf;       // work-groups of the local sort
     int huge;       // work-groups of the large-leaf sort (one per CU)
+    int tilepart;   // work-groups of the tile-local partition (one per CU)
 };
 
 // Launchers (all stream-ordered, no host synchronisation).
@@ -85,8 +84,10 @@ struct Grid {
 constexpr uint32_t kInitLeaf = 0x100u, kInitHuge = 0x200u;
 // Zeroes ctl[0, nblocks) except, with keep_status, the sticky status word
 // ctl[kSeedCtl].overflow (later runs of one call keep what earlier runs set).
-hipError_t launch_init(LevelCtl* ctl, int nblocks, bool keep_status, SegRec* seg0, uint32_t seg0_count, uint64_t n,
-                       uint32_t flags, hipStream_t st);
+// The single range [0, n) goes to seg0 (a segment to partition) or, with
+// kInitLeaf / kInitHuge, to leaf0 (the leaf list or the large-leaf list).
+hipError_t launch_init(LevelCtl* ctl, int nblocks, bool keep_status, SegRec* seg0, LeafRec* leaf0,
+                       uint32_t seg0_count, uint64_t n, uint32_t flags, hipStream_t st);
 hipError_t launch_leaf(int dtype, const LeafArgs& a, const Grid& g, hipStream_t st);
 // Launchers of the key-type kernels (kernels_typed.hip, one instantiation per
 // key type): level steps 0 (splitters), 2 (count) and 6 (scatter), and the
@@ -102,8 +103,8 @@ struct SeedCaps {
     uint32_t segs, leaves, huge;  // entries of the level-0 queue, the leaf list and the large-leaf list
 };
 hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev,
-                                uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves,
-                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, SeedCaps caps, hipStream_t st);
+                                uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, LeafRec* leaves,
+                                uint32_t leaf_max, uint32_t leaf_cap, LeafRec* huge, SeedCaps caps, hipStream_t st);
 
 // Merge of two sorted runs a[0, na) and b[0, nb) into out[0, na + nb) (out
 // overlaps neither run).  `split` holds merge_split_entries(dtype, na + nb)

@@ -71,8 +71,8 @@ __device__ __forceinline__ void flag_overflow(LevelCtl* ctl, uint32_t* status, u
     atomicOr(status, bits);
 }
 
-__global__ void k_init(LevelCtl* ctl, int nblocks, uint32_t keep_status, SegRec* seg0, uint32_t seg0_count, uint64_t n,
-                       uint32_t flags) {
+__global__ void k_init(LevelCtl* ctl, int nblocks, uint32_t keep_status, SegRec* seg0, LeafRec* leaf0,
+                       uint32_t seg0_count, uint64_t n, uint32_t flags) {
     constexpr uint32_t kWords = (uint32_t)(sizeof(LevelCtl) / 4);
     const uint32_t words = (uint32_t)nblocks * kWords;
     const uint32_t status_word = (uint32_t)kSeedCtl * kWords + (uint32_t)(offsetof(LevelCtl, overflow) / 4);
@@ -81,10 +81,14 @@ __global__ void k_init(LevelCtl* ctl, int nblocks, uint32_t keep_status, SegRec*
         if (!keep_status || i != status_word) w[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0 && seg0_count) {
-        seg0[0] = SegRec{0, (uint32_t)n, flags & 0xFFu};
-        if (flags & kInitHuge) ctl[kSeedCtl].nhuge = 1;  // the single range is a large leaf
-        else if (flags & kInitLeaf) ctl[kSeedCtl].nleaf_list = 1;  // ... a leaf
-        else ctl[0].nseg = 1;
+        if (flags & (kInitHuge | kInitLeaf)) {
+            leaf0[0] = LeafRec{0, (uint32_t)n, flags & 0xFFu, 0, 0, 0};
+            if (flags & kInitHuge) ctl[kSeedCtl].nhuge = 1;  // the single range is a large leaf
+            else ctl[kSeedCtl].nleaf_list = 1;               // ... a leaf
+        } else {
+            seg0[0] = SegRec{0, (uint32_t)n, flags & 0xFFu};
+            ctl[0].nseg = 1;
+        }
     }
 }
 
@@ -95,8 +99,8 @@ __global__ void k_init(LevelCtl* ctl, int nblocks, uint32_t keep_status, SegRec*
 // seed block's lists, so that level 0's own large leaves (appended to
 // ctl0->nhuge by its plan) do not share a counter with them.
 __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev, uint64_t total_keys,
-                       SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves, uint32_t leaf_max,
-                       uint32_t leaf_cap, SegRec* huge, SeedCaps caps) {
+                       SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, LeafRec* leaves, uint32_t leaf_max,
+                       uint32_t leaf_cap, LeafRec* huge, SeedCaps caps) {
     uint32_t nseg = nseg_in;
     if (nseg_dev) {
         nseg = *nseg_dev;
@@ -115,18 +119,22 @@ __global__ void k_seed(const uint64_t* offsets, uint32_t nseg_in, const uint32_t
         if (len < 2) continue;
         // Disjoint arrays within total_keys never exceed the capacities (they
         // are sized from total_keys); the checks keep any other input in bounds.
-        SegRec r{lo, (uint32_t)len, SEG_RAW};
-        uint32_t slot, cap;
-        SegRec* list;
+        uint32_t slot;
+        bool ok;
         if (len > leaf_max) {
-            slot = atomicAdd(&ctl0->nseg, 1u), cap = caps.segs, list = segs;
-        } else if (len > leaf_cap) {
-            slot = atomicAdd(&seed->nhuge, 1u), cap = caps.huge, list = huge;
+            slot = atomicAdd(&ctl0->nseg, 1u);
+            if ((ok = slot < caps.segs)) segs[slot] = SegRec{lo, (uint32_t)len, SEG_RAW};
         } else {
-            slot = atomicAdd(&seed->nleaf_list, 1u), cap = caps.leaves, list = leaves;
+            const LeafRec r{lo, (uint32_t)len, SEG_RAW, 0, 0, 0};
+            if (len > leaf_cap) {
+                slot = atomicAdd(&seed->nhuge, 1u);
+                if ((ok = slot < caps.huge)) huge[slot] = r;
+            } else {
+                slot = atomicAdd(&seed->nleaf_list, 1u);
+                if ((ok = slot < caps.leaves)) leaves[slot] = r;
+            }
         }
-        if (slot < cap) list[slot] = r;
-        else atomicOr(&seed->overflow, kOvfSegs);
+        if (!ok) atomicOr(&seed->overflow, kOvfSegs);
     }
 }
 
@@ -150,8 +158,10 @@ __global__ __launch_bounds__(kPlanWG) void k_tileplan(LevelArgs a) {
             keys += a.segs[s].n;
         }
         uint32_t tt, tm, tb;
+        // a tile-local level stores one row more per segment: the tile sizes
+        const uint32_t rows = nb + (a.tile_local ? 1u : 0u);
         const uint32_t et = block_excl_scan<kPlanWG>(nt, wsum, &tt);
-        const uint32_t em = block_excl_scan<kPlanWG>(nt * nb, wsum, &tm);
+        const uint32_t em = block_excl_scan<kPlanWG>(nt * rows, wsum, &tm);
         const uint32_t eb = block_excl_scan<kPlanWG>(nb, wsum, &tb);
         if (s < nseg) {
             a.plans[s].tile_begin = ct + et;
@@ -297,8 +307,9 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
             const uint32_t rel1 = (b + 1 < p.nbuckets) ? a.mat[p.mat_base + (b + 1) * p.ntiles] - m0 : seg.n;
             size = rel1 - rel;
             eq = eqi[b] != 0 && size > 0;
-            a.bkt[p.bucket_base + b] =
-                BucketRec{seg.start + rel, size | (eq ? kBucketDirect : 0u), (seg.flags & SEG_SRC_DN) ? 0u : 1u};
+            // the scatter wrote the keys to the other buffer
+            a.bkt[p.bucket_base + b] = LeafRec{seg.start + rel, size | (eq ? kBucketDirect : 0u),
+                                               (seg.flags & SEG_SRC_DN) ? 0u : SEG_SRC_DN, 0, 0, 0};
             if (eq) nfill = (size + kFillChunk - 1) / kFillChunk;
             if (!eq && size > a.leaf_max && !a.is_last) {
                 const uint32_t slot = atomicAdd(&a.ctl_next->nseg, 1u);
@@ -311,7 +322,7 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
                 // a leaf too large for one 512-thread work-group: the large-leaf list
                 const uint32_t slot = atomicAdd(&a.ctl->nhuge, 1u);
                 if (slot < a.huge_cap) {
-                    a.huge[slot] = SegRec{seg.start + rel, size, (seg.flags & SEG_SRC_DN) ^ SEG_SRC_DN};
+                    a.huge[slot] = LeafRec{seg.start + rel, size, (seg.flags & SEG_SRC_DN) ^ SEG_SRC_DN, 0, 0, 0};
                 } else {
                     flag_overflow(a.ctl, a.status, kOvfHuge);
                 }
@@ -338,17 +349,90 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
 
 
 
+// ------------------------------------------------------------ k_plan_local
+// Plan of a tile-local level (after k_tilepart), one work-group per segment,
+// one thread per bucket.  k_tilepart left in the count matrix, for every tile
+// t of the segment, the in-tile start S(b, t) of each bucket and, as row nb,
+// the tile's size, and added every tile's count of bucket b to bsum; the
+// exclusive scan of bsum gives each bucket's place in the sorted segment.
+// Per bucket: a gathered leaf record (final position in d, the tiles'
+// base in dn, its row of in-tile starts), the large-leaf list for buckets
+// above one 512-thread work-group's capacity, fill tasks for equality
+// buckets, and a rescue segment (partitioned by the next level from d, where
+// the leaf pass first gathers it) for a bucket above leaf_max.
+__global__ __launch_bounds__(256) void k_plan_local(LevelArgs a) {
+    __shared__ uint16_t eqi[kSplStride + 1];  // splitter index + 1 of an equality bucket, else 0
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t fbase;
+    __shared__ uint32_t rsum[kSplStride + 1];  // bucket starts in the segment, then its size
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
+    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const SegRec seg = a.segs[s];
+        const SegPlan p = a.plans[s];
+        for (uint32_t i = threadIdx.x; i <= (uint32_t)kSplStride; i += 256) eqi[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kSplStride; i += 256) {
+            const uint16_t m = a.meta[(size_t)s * kSplStride + i];
+            if (m >> 8) eqi[(m & 0xFFu) + 1] = (uint16_t)(i + 1);
+        }
+        // bucket sizes (k_tilepart added every tile's counts) and their
+        // exclusive scan: where each bucket starts in the sorted segment
+        const uint32_t b = threadIdx.x;
+        const uint32_t bs = (b < p.nbuckets) ? a.bsum[(size_t)s * kSplStride + b] : 0u;
+        uint32_t stot;
+        rsum[b] = block_excl_scan<256>(bs, wsum, &stot);
+        if (b == 0) rsum[p.nbuckets] = stot;
+        __syncthreads();
+        uint32_t rel = 0, size = 0, nfill = 0;
+        if (b < p.nbuckets) {
+            rel = rsum[b];
+            size = rsum[b + 1] - rel;
+            const bool eq = eqi[b] != 0 && size > 0;
+            a.bkt[p.bucket_base + b] = LeafRec{seg.start + rel, size | (eq ? kBucketDirect : 0u), SEG_GATHER,
+                                               seg.start, p.mat_base + b * p.ntiles, p.ntiles};
+            if (eq) {
+                nfill = (size + kFillChunk - 1) / kFillChunk;
+            } else if (size > a.leaf_max) {
+                // gathered into d by the leaf pass, then partitioned again
+                const uint32_t slot = atomicAdd(&a.ctl_next->nseg, 1u);
+                if (slot < a.seg_cap) a.next_segs[slot] = SegRec{seg.start + rel, size, 0u};
+                else flag_overflow(a.ctl_next, a.status, kOvfSegs);
+            } else if (size > a.leaf_cap) {
+                const uint32_t slot = atomicAdd(&a.ctl->nhuge, 1u);
+                if (slot < a.huge_cap) a.huge[slot] = a.bkt[p.bucket_base + b];
+                else flag_overflow(a.ctl, a.status, kOvfHuge);
+            }
+        }
+        uint32_t ftotal;
+        const uint32_t fex = block_excl_scan<256>(nfill, wsum, &ftotal);
+        if (threadIdx.x == 0 && ftotal) fbase = atomicAdd(&a.ctl->nfill, ftotal);
+        __syncthreads();
+        if (nfill) {
+            const uint32_t spl_index = (uint32_t)s * kSplStride + (eqi[b] - 1u);
+            for (uint32_t c = 0; c < nfill; ++c) {
+                const uint32_t slot = fbase + fex + c;
+                const uint32_t off = c * kFillChunk;
+                if (slot < a.fill_cap)
+                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), spl_index};
+                else
+                    flag_overflow(a.ctl, a.status, kOvfFill);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ launchers
-hipError_t launch_init(LevelCtl* ctl, int nblocks, bool keep_status, SegRec* seg0, uint32_t seg0_count, uint64_t n,
-                       uint32_t flags, hipStream_t st) {
-    hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, st, ctl, nblocks, keep_status ? 1u : 0u, seg0, seg0_count, n,
-                       flags);
+hipError_t launch_init(LevelCtl* ctl, int nblocks, bool keep_status, SegRec* seg0, LeafRec* leaf0,
+                       uint32_t seg0_count, uint64_t n, uint32_t flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, st, ctl, nblocks, keep_status ? 1u : 0u, seg0, leaf0, seg0_count,
+                       n, flags);
     return hipGetLastError();
 }
 
 hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const uint32_t* nseg_dev,
-                                uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, SegRec* leaves,
-                                uint32_t leaf_max, uint32_t leaf_cap, SegRec* huge, SeedCaps caps, hipStream_t st) {
+                                uint64_t total_keys, SegRec* segs, LevelCtl* ctl0, LevelCtl* seed, LeafRec* leaves,
+                                uint32_t leaf_max, uint32_t leaf_cap, LeafRec* huge, SeedCaps caps, hipStream_t st) {
     const uint32_t blocks = std::max(1u, std::min(1024u, (nseg_in + 255) / 256));
     hipLaunchKernelGGL(k_seed, dim3(blocks), dim3(256), 0, st, offsets, nseg_in, nseg_dev, (unsigned long long)total_keys,
                        segs, ctl0, seed, leaves, leaf_max, leaf_cap, huge, caps);
@@ -361,7 +445,8 @@ hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid
         case 3: hipLaunchKernelGGL(k_scan_reduce, dim3(g.scan), dim3(256), 0, st, a); return hipGetLastError();
         case 4: hipLaunchKernelGGL(k_scan_apply, dim3(g.scan), dim3(256), 0, st, a); return hipGetLastError();
         case 5: hipLaunchKernelGGL(k_plan, dim3(g.plan), dim3(256), 0, st, a); return hipGetLastError();
-        case 0: case 2: case 6: break;  // typed: splitters, count, scatter
+        case 8: hipLaunchKernelGGL(k_plan_local, dim3(g.plan), dim3(256), 0, st, a); return hipGetLastError();
+        case 0: case 2: case 6: case 7: break;  // typed: splitters, count, scatter, tilepart
         default: return hipErrorInvalidValue;
     }
     switch (dtype) {

@@ -18,6 +18,7 @@ hipError_t typed_level_step(int step, const LevelArgs& a, const Grid& g, hipStre
             hipLaunchKernelGGL(k_scatter<DT>, dim3(std::max(1, g.part * kPartWG / kScatterWG)), dim3(kScatterWG), 0, st,
                                a);
             break;
+        case 7: hipLaunchKernelGGL(k_tilepart<DT>, dim3(g.tilepart), dim3(kScatterWG), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -26,14 +27,15 @@ hipError_t typed_level_step(int step, const LevelArgs& a, const Grid& g, hipStre
 template <int DT>
 hipError_t typed_leaf(const LeafArgs& a, const Grid& g, hipStream_t st) {
     hipLaunchKernelGGL((k_leaf_small<DT>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
-    if (a.leaf_max > a.bitonic_cutoff || a.is_last)
-        hipLaunchKernelGGL((k_leaf_big<DT, false>), dim3(g.leaf), dim3(kLeafBigWG), 0, st, a);
+    // gathered leaves (a tile-local level) are never sorted by one wave
+    if (a.leaf_max > a.bitonic_cutoff || a.is_last || a.lv.tile_local)
+        hipLaunchKernelGGL((k_leaf_big<DT, 512>), dim3(g.leaf), dim3(512), 0, st, a);
     if (a.leaf_max > a.leaf_cap) {  // leaves above the 512-thread capacity: the large-leaf list
         LeafArgs h = a;
         h.list = a.huge;
         h.list_count = a.huge_count;
         h.list_cap = a.lv.huge_cap;
-        hipLaunchKernelGGL((k_leaf_big<DT, true>), dim3(g.huge), dim3(2 * kLeafBigWG), 0, st, h);
+        hipLaunchKernelGGL((k_leaf_big<DT, 1024>), dim3(g.huge), dim3(1024), 0, st, h);
     }
     return hipGetLastError();
 }

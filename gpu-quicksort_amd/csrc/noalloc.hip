@@ -45,12 +45,13 @@ size_t carve(Workspace& w, unsigned char* base, const Caps& c) {
     take(&w.meta, c.seg * kSplStride, sizeof(uint16_t));
     take(&w.lut, c.seg * kLutCells, sizeof(uint8_t));
     take(&w.mat, c.mat, sizeof(uint32_t));
+    take(&w.bsum, c.seg * kSplStride, sizeof(uint32_t));
     take(&w.part, c.part, sizeof(uint32_t));
-    take(&w.bkt, c.bkt, sizeof(BucketRec));
+    take(&w.bkt, c.bkt, sizeof(LeafRec));
     take(&w.fill, c.fill, sizeof(FillTask));
     take(&w.ctl, kCtlBlocks, sizeof(LevelCtl));
-    take(&w.leaves, c.leaves, sizeof(SegRec));
-    take(&w.huge, c.huge, sizeof(SegRec));
+    take(&w.leaves, c.leaves, sizeof(LeafRec));
+    take(&w.huge, c.huge, sizeof(LeafRec));
     w.cap = c;
     w.key_bytes = 8;
     return off;

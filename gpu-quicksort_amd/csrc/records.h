@@ -29,12 +29,30 @@ enum Mode : int { MODE_MULTIPIVOT = 0, MODE_CLASSIC = 1 };
 // SegRec::flags
 constexpr uint32_t SEG_SRC_DN = 1u;  // keys live in dn (else in d)
 constexpr uint32_t SEG_RAW = 2u;     // keys are still in caller representation
+constexpr uint32_t SEG_GATHER = 4u;  // a leaf whose keys are runs of a tile-local level's tiles
 
 // A key range [start, start+n) to partition (or to sort locally).
 struct SegRec {
     uint64_t start;
     uint32_t n;
     uint32_t flags;
+};
+
+// A leaf to sort locally: an entry of the level's bucket table, of the list
+// of arrays a sort starts from, or of the large-leaf list.  A contiguous leaf
+// is keys [start, start + n) of d (or of dn with SEG_SRC_DN).  A gathered
+// leaf (SEG_GATHER) is one bucket b of a segment partitioned by a tile-local
+// level: tile t of the segment (kTileKeys keys of dn from src + t * kTileKeys)
+// holds the bucket's keys at [S(b, t), S(b + 1, t)), where S(b, .) is the row
+// mat[row ...] of ntiles in-tile starts and S(b + 1, .) the row after it; the
+// sorted leaf goes to d[start, start + n).
+struct LeafRec {
+    uint64_t start;   // first key (contiguous) or final position (gathered)
+    uint32_t n;       // keys; in the bucket table bit 31 marks an equality bucket
+    uint32_t flags;   // SEG_SRC_DN, SEG_RAW, SEG_GATHER
+    uint64_t src;     // gathered: first key of the segment's tiles in dn
+    uint32_t row;     // gathered: count-matrix index of S(b, 0)
+    uint32_t ntiles;  // gathered: tiles of the segment
 };
 
 // Per-segment plan of one level, written by the splitter and tile-plan kernels.

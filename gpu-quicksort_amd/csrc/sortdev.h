@@ -500,6 +500,8 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
         const uint32_t n = seg.n;
         K* spl = (K*)a.spl + (size_t)s * kSplStride;
         uint16_t* meta = a.meta + (size_t)s * kSplStride;
+        if (a.tile_local)
+            for (uint32_t i = tid; i < (uint32_t)kSplStride; i += kSplWG) a.bsum[(size_t)s * kSplStride + i] = 0;
         // Candidate splitters cand[0..ncand), sorted, possibly with repeats.
         uint32_t ncand;
         if (a.mode == MODE_CLASSIC) {
@@ -949,7 +951,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
             const uint32_t* row = &mblk[ti * kRow + (tl - mb_lo)];
             my_cursor = row[0] - mb_base;
             mycnt = (ti + 1 < p.nbuckets) ? row[1] - row[0] : 0u;  // the last run needs no count
-            bsize = a.bkt[p.bucket_base + ti].size;
+            bsize = a.bkt[p.bucket_base + ti].n;
         }
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         K* dst = ((seg.flags & SEG_SRC_DN) ? (K*)a.d : (K*)a.dn) + seg.start;
@@ -1076,6 +1078,164 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     }
 }
 
+// ------------------------------------------------------------ k_tilepart
+// The tile-local partition level (the last level before the local sorts):
+// persistent over tiles like k_scatter, each work-group a contiguous run of
+// tiles.  Per tile: classify the keys, rank them within their bucket by
+// returning LDS adds, scan the counts into the in-tile bucket starts S(b, t),
+// stage the tile grouped by bucket and store it over the same positions of dn
+// (in place when the segment already lives in dn: the whole tile is in
+// registers before any of it is stored).  S(., t) and, as row nb, the tile's
+// size go to the count matrix for k_plan_local and for the leaf pass, which
+// gathers every bucket's runs from the tiles.  So this level reads and writes
+// each key once, with no count pass, and its stores are whole tiles.  Equality
+// buckets are stored too (the leaf pass fills their value into d).
+template <int DT>
+__global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
+    using K = typename KeyOf<DT>::K;
+    using TS = TileShape<K>;
+    constexpr int KPT = TS::kKeysPerThread;
+    constexpr uint32_t kTile = (uint32_t)kScatterWG * KPT;
+    static_assert(TS::kTileKeys == (int)kTile, "one tile per pass of the work-group");
+    static_assert(kTile <= 65536 && kSplStride <= 256, "bucket and rank share one word");
+    constexpr bool kEarlyPrefetch = DT == DT_U32;  // keys need no transform
+    __shared__ TileLds<K> L;
+    __shared__ K stage[kTile];
+    __shared__ uint32_t wsum[kScatterWG / 64];
+    __shared__ SegRec s_seg;
+    __shared__ SegPlan s_plan;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t total = a.ctl->total_tiles;
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
+    uint32_t seg_i = 0, seg_next = 0;
+    if (tid == 0 && t0 < t1) {
+        seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
+        seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+    }
+    uint32_t cur = 0xFFFFFFFFu;
+    K keys[KPT];
+    bool have_keys = false;  // keys[] already hold this tile's keys
+    for (uint32_t t = t0; t < t1; ++t) {
+        // tid, opaque to the compiler (LDS addresses derived from it are
+        // recomputed per tile instead of being held across the loop)
+        uint32_t ti;
+        asm volatile("" : "=v"(ti) : "0"(tid));
+        if (ti == 0) {
+            while (t >= seg_next) {
+                ++seg_i;
+                seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+            }
+            L.seg = seg_i;
+        }
+        __syncthreads();  // the previous tile's stores have read stage
+        const uint32_t s = uni(L.seg);
+        if (s != cur) {
+            load_splitters<K>(a, s, L);
+            if (ti == 0) {
+                s_seg = a.segs[s];
+                s_plan = a.plans[s];
+            }
+            cur = s;
+            __syncthreads();
+        }
+        const SegRec seg = uni(s_seg);
+        const SegPlan p = uni(s_plan);
+        const uint32_t tl = t - p.tile_begin;
+        if (ti < (uint32_t)kSplStride) L.hist[ti] = 0;
+        const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
+        K* dst = (K*)a.dn + seg.start;
+        const bool raw = seg.flags & SEG_RAW;
+        const uint32_t lo = tl * kTile;
+        const uint32_t hi = min(lo + kTile, seg.n);
+        const uint32_t cnt = hi - lo;
+        const K* base = src + seg.start;
+        if (!have_keys) {
+            K raw_keys[KPT];
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) raw_keys[j] = base[min(lo + j * kScatterWG + ti, hi - 1u)];
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const K v = raw ? to_key<DT>(raw_keys[j]) : raw_keys[j];
+                keys[j] = (lo + j * kScatterWG + ti < hi) ? v : K(0);
+            }
+        }
+        __syncthreads();  // counters cleared
+        // bucket (< 256) and rank within the bucket (< kTile), packed
+        uint32_t br[KPT];
+        {
+            uint32_t bk[KPT], rk[KPT];
+            bool valid[KPT];
+            classify_lut<K, KPT>(keys, bk, L, p);
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) valid[j] = lo + j * kScatterWG + ti < hi;
+            hist_rank_n<KPT>(L.hist, bk, valid, rk);
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
+        }
+        __syncthreads();  // every count is in hist
+        // in-tile bucket starts: thread b owns bucket b
+        {
+            const uint32_t mycnt = (ti < p.nbuckets) ? L.hist[ti] : 0u;
+            uint32_t tot;
+            const uint32_t st = block_excl_scan<kScatterWG, false, false>(mycnt, wsum, &tot);
+            uint32_t* col = a.mat + p.mat_base + tl;
+            if (ti < p.nbuckets) {
+                L.hist[ti] = st;
+                col[(size_t)ti * p.ntiles] = st;
+                if (mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
+            } else if (ti == p.nbuckets) {
+                col[(size_t)ti * p.ntiles] = cnt;  // row nb: the tile's size
+            }
+        }
+        // prefetch: the next tile when it is this work-group's and in the same
+        // segment (uint32 keys now, float keys after the staging, as k_scatter)
+        const bool next_tile = t + 1 < t1 && tl + 1 < p.ntiles;
+        const uint32_t nlo = lo + kTile;
+        const uint32_t nhi = next_tile ? min(nlo + kTile, seg.n) : 0u;
+        K nxt[KPT];
+        if constexpr (kEarlyPrefetch) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const uint32_t idx = nlo + j * kScatterWG + ti;
+                nxt[j] = (idx < nhi) ? base[idx] : K(0);
+            }
+        }
+        __syncthreads();  // bucket starts in hist
+#pragma unroll
+        for (int j = 0; j < KPT; ++j)
+            if (lo + j * kScatterWG + ti < hi) stage[L.hist[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
+        if constexpr (!kEarlyPrefetch) {
+            if (next_tile) {  // uniform
+#pragma unroll
+                for (int j = 0; j < KPT; ++j) nxt[j] = base[min(nlo + j * kScatterWG + ti, nhi - 1u)];
+            } else {
+#pragma unroll
+                for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
+            }
+        }
+        __syncthreads();  // the tile is staged
+        // the tile back, grouped by bucket: coalesced stores of whole tiles
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            const uint32_t i = j * kScatterWG + ti;
+            if (i < cnt) dst[lo + i] = stage[i];
+        }
+        if constexpr (kEarlyPrefetch) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const K v = raw ? to_key<DT>(nxt[j]) : nxt[j];
+                keys[j] = (nlo + j * kScatterWG + ti < nhi) ? v : K(0);
+            }
+        }
+        have_keys = next_tile;
+    }
+}
+
 // ------------------------------------------------------------ k_leaf
 // Local sort of one leaf (the reference's lqsort_kernel,
 // CL12/QuicksortKernels.cl:267-421, with sort_threshold/bitonic_sort as its
@@ -1091,7 +1251,6 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
 //     odd-even merge network; buckets above the register capacity go to one
 //     wave each (bitonic) or, above bitonic_cutoff (rare), to the whole
 //     work-group in LDS.
-constexpr int kLeafBigWG = 512;
 // largest bucket one thread sorts in registers (64 VGPRs of keys at most)
 template <class K>
 constexpr uint32_t lane_sort_max() { return sizeof(K) == 4 ? 32u : 16u; }
@@ -1343,47 +1502,144 @@ __device__ __forceinline__ void wave_sort_any(uint32_t n, LD ld, ST st) {
 }
 
 struct LeafDesc {
-    uint64_t start;
+    uint64_t start;   // contiguous: first key; gathered: final position in d
+    uint64_t src;     // gathered: first key of the segment's tiles in dn
     uint32_t n;
-    uint32_t kind;  // 0 none, 1 wave, 2 work-group, 3 unbounded, 4 large (1024-thread work-group)
+    uint32_t kind;    // 0 none, 1 wave, 2 512-thread work-group, 3 unbounded, 4 large (1024-thread
+                      // work-group), 5 an oversize gathered bucket: copied to d in order for the
+                      // next level
     uint32_t from_dn;
     uint32_t raw;
+    uint32_t gather;  // runs of a tile-local level's tiles (see LeafRec)
+    uint32_t row;     // gathered: count-matrix index of S(b, 0)
+    uint32_t ntiles;  // gathered: tiles of the segment
 };
 
-// Entry idx of the leaf space as 16 raw bytes: a SegRec of the level-0 list
-// or a BucketRec of the level (both: 64-bit start, 32-bit size, 32-bit
-// flags).  Kept raw so that its load can stay in flight until decoded.
-static_assert(sizeof(SegRec) == 16 && sizeof(BucketRec) == 16, "leaf records are 16 bytes");
-__device__ __forceinline__ uint4 leaf_rec(const LeafArgs& a, bool list_mode, uint32_t idx, uint32_t count) {
-    if (idx >= count) return make_uint4(0u, 0u, 0u, 0u);
-    return list_mode ? reinterpret_cast<const uint4*>(a.list)[idx] : reinterpret_cast<const uint4*>(a.lv.bkt)[idx];
+// Entry idx of the leaf space as 32 raw bytes (a LeafRec of a leaf list or of
+// the level's bucket table), kept raw so that its loads can stay in flight
+// until decoded.
+struct LeafRaw {
+    uint4 lo, hi;
+};
+static_assert(sizeof(LeafRec) == 32, "leaf records are 32 bytes");
+__device__ __forceinline__ LeafRaw leaf_rec(const LeafArgs& a, bool list_mode, uint32_t idx, uint32_t count) {
+    if (idx >= count) return LeafRaw{make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+    const uint4* q = list_mode ? reinterpret_cast<const uint4*>(a.list) : reinterpret_cast<const uint4*>(a.lv.bkt);
+    return LeafRaw{q[2 * (size_t)idx], q[2 * (size_t)idx + 1]};
 }
-__device__ __forceinline__ LeafDesc leaf_decode(const LeafArgs& a, bool list_mode, uint4 r) {
-    LeafDesc ds{0, 0, 0, 0, 0};
-    const uint64_t start = (uint64_t)r.x | ((uint64_t)r.y << 32);
-    if (list_mode) {
-        ds.start = start;
-        ds.n = r.z;
-        ds.from_dn = (r.w & SEG_SRC_DN) ? 1u : 0u;
-        ds.raw = (r.w & SEG_RAW) ? 1u : 0u;
-    } else if (!(r.z & kBucketDirect) && (r.z <= a.leaf_max || a.is_last)) {
-        ds.start = start;
-        ds.n = r.z;
-        ds.from_dn = r.w;
+__device__ __forceinline__ LeafDesc leaf_decode(const LeafArgs& a, bool list_mode, const LeafRaw& r) {
+    LeafDesc ds{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t n = r.lo.z, flags = r.lo.w;
+    const bool gather = (flags & SEG_GATHER) != 0;
+    // a level's bucket table: equality buckets are filled, buckets above
+    // leaf_max of a contiguous level are the next level's segments
+    const bool take = list_mode || (!(n & kBucketDirect) && (n <= a.leaf_max || a.is_last || gather));
+    if (take && n >= 1) {
+        ds.start = (uint64_t)r.lo.x | ((uint64_t)r.lo.y << 32);
+        ds.n = n;
+        ds.from_dn = (flags & SEG_SRC_DN) ? 1u : 0u;
+        ds.raw = (flags & SEG_RAW) ? 1u : 0u;
+        ds.gather = gather ? 1u : 0u;
+        ds.src = (uint64_t)r.hi.x | ((uint64_t)r.hi.y << 32);
+        ds.row = r.hi.z;
+        ds.ntiles = r.hi.w;
+        if (gather) ds.kind = n <= a.leaf_cap ? 2u : (n <= a.leaf_max ? 4u : 5u);
+        else ds.kind = (n <= a.bitonic_cutoff) ? 1u : (n <= a.leaf_cap ? 2u : (n <= a.leaf_max ? 4u : 3u));
     }
-    if (ds.n >= 1)
-        ds.kind = (ds.n <= a.bitonic_cutoff) ? 1u : (ds.n <= a.leaf_cap ? 2u : (ds.n <= a.leaf_max ? 4u : 3u));
     return ds;
 }
 __device__ __forceinline__ LeafDesc uniform_desc(const LeafDesc& d) {
     LeafDesc u;
-    u.start = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(d.start >> 32)) << 32) |
-              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)d.start);
-    u.n = __builtin_amdgcn_readfirstlane(d.n);
-    u.kind = __builtin_amdgcn_readfirstlane(d.kind);
-    u.from_dn = __builtin_amdgcn_readfirstlane(d.from_dn);
-    u.raw = __builtin_amdgcn_readfirstlane(d.raw);
+    u.start = uni(d.start);
+    u.src = uni(d.src);
+    u.n = uni(d.n);
+    u.kind = uni(d.kind);
+    u.from_dn = uni(d.from_dn);
+    u.raw = uni(d.raw);
+    u.gather = uni(d.gather);
+    u.row = uni(d.row);
+    u.ntiles = uni(d.ntiles);
     return u;
+}
+
+// Rows of a gathered leaf staged in LDS ahead of its copy: S(b, t) for the
+// first kRowsLds tiles, then S(b + 1, t).
+constexpr uint32_t kRowsLds = 64;
+
+// Asynchronous copy of a gathered leaf into LDS (global_load_lds, no
+// registers): run t of the leaf is S(b, t) .. S(b+1, t) of tile t, the rows
+// rowb[0, ntiles) and rowb[ntiles, 2 * ntiles), read from rows_lds (staged
+// there earlier) when given and ntiles <= kRowsLds.  Every wave reads the rows
+// of 64 tiles at a time, scans the run lengths itself (DPP) and copies the
+// runs of the tiles t = wave (mod waves), 64 dwords per instruction, to their
+// offsets in dst.  Complete after vm_wait_all() in every wave and a barrier.
+template <class K, int NT>
+__device__ __forceinline__ void gather_to_lds_async(K* dst, const K* base, const uint32_t* rowb, uint32_t ntiles,
+                                                    const uint32_t* rows_lds, uint32_t tid) {
+    constexpr uint32_t NW = NT / 64, DW = sizeof(K) / 4;
+    constexpr uint32_t kTile = TileShape<K>::kTileKeys;
+    const uint32_t lane = tid & 63u, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(base);
+    uint32_t* p = reinterpret_cast<uint32_t*>(dst);
+    uint32_t off = 0;  // keys of the chunks before this one
+    for (uint32_t c0 = 0; c0 < ntiles; c0 += 64) {
+        const uint32_t t = c0 + lane;
+        uint32_t s0 = 0, len = 0;
+        if (t < ntiles) {
+            if (rows_lds && ntiles <= kRowsLds) {
+                s0 = rows_lds[t];
+                len = rows_lds[kRowsLds + t] - s0;
+            } else {
+                s0 = rowb[t];
+                len = rowb[ntiles + t] - s0;
+            }
+        }
+        const uint32_t incl = wave_incl_scan(len);
+        const uint32_t excl = incl - len;
+        for (uint32_t l = w; l < 64u && c0 + l < ntiles; l += NW) {
+            const uint32_t rs0 = (uint32_t)__builtin_amdgcn_readlane((int)s0, (int)l);
+            const uint32_t ndw = (uint32_t)__builtin_amdgcn_readlane((int)len, (int)l) * DW;
+            const uint32_t roff = (uint32_t)__builtin_amdgcn_readlane((int)excl, (int)l);
+            const uint32_t* src = s + ((size_t)(c0 + l) * kTile + rs0) * DW;
+            uint32_t* d = p + (size_t)(off + roff) * DW;
+            for (uint32_t q = 0; q < ndw; q += 64)
+                if (q + lane < ndw)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + q + lane),
+                                                     (__attribute__((address_space(3))) void*)(d + q), 4, 0, 0);
+        }
+        off += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+}
+
+// Wave 0 stages the two rows of a gathered leaf of at most kRowsLds tiles in
+// rows_lds with asynchronous copies (complete after vm_wait_all() in wave 0
+// and a barrier).
+__device__ __forceinline__ void stage_rows_async(uint32_t* rows_lds, const uint32_t* rowb, uint32_t ntiles,
+                                                 uint32_t tid) {
+    if (tid < 64 && ntiles <= kRowsLds) {
+        const uint32_t lane = tid;
+        if (lane < ntiles) {
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowb + lane),
+                                             (__attribute__((address_space(3))) void*)(rows_lds), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowb + ntiles + lane),
+                                             (__attribute__((address_space(3))) void*)(rows_lds + kRowsLds), 4, 0, 0);
+        }
+    }
+}
+
+// The same runs copied in order to out[0, n) in global memory (an oversize
+// gathered bucket, partitioned again by the next level); one work-group, run
+// by run.
+template <class K, int NT>
+__device__ void gather_copy_global(K* out, const K* base, const uint32_t* rowb, uint32_t ntiles, uint32_t tid) {
+    constexpr uint32_t kTile = TileShape<K>::kTileKeys;
+    uint32_t off = 0;
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const uint32_t s0 = uni(rowb[t]), s1 = uni(rowb[ntiles + t]);
+        const K* src = base + (size_t)t * kTile + s0;
+        for (uint32_t i = tid; i < s1 - s0; i += NT) out[off + i] = src[i];
+        off += s1 - s0;
+    }
 }
 
 // Leaves of at most bitonic_cutoff keys: one per wave, four per work-group of
@@ -1445,8 +1701,8 @@ __device__ __forceinline__ void copy_to_lds_async(K* dst, const K* src, uint32_t
     }
 }
 
-// Leaves above bitonic_cutoff keys, one per work-group of kLeafBigWG threads,
-// software-pipelined over the work-group's leaves g_i = blockIdx.x + i * G:
+// Leaves above bitonic_cutoff keys, one per work-group of NTX threads,
+// software-pipelined over the work-group's leaves g_i = g0 + i * G:
 // while leaf g_i is sorted (keys in registers, LDS buffer A), the keys of
 // g_{i+1} stream into LDS buffer P, the record of g_{i+2} is in flight, and
 // the sorted g_{i-1} is being written out from A.  Barriers inside the
@@ -1460,19 +1716,24 @@ __device__ __forceinline__ void copy_to_lds_async(K* dst, const K* src, uint32_t
 // register capacity (skewed or duplicate-heavy leaves) go to one wave each,
 // or to the work-group in LDS.
 //
-// kHuge: the same kernel with 1024 threads, twice the capacity and one local
-// bucket per thread, for the few leaves between the 512-thread capacity and
-// leaf_max (kind 4, read from the large-leaf list), so that buckets slightly
-// above that capacity need no further partition level.
-template <int DT, bool kHuge>
-__global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf_big(LeafArgs a) {
+// One kernel in two sizes, 16 (double: 8) keys per thread and one local
+// bucket per thread: NTX = 512 for leaves up to leaf_cap (kind 2; it also
+// takes the unbounded fallback, kind 3, and the copies of oversize gathered
+// buckets, kind 5); NTX = 1024 for the leaves between leaf_cap and leaf_max
+// (kind 4, read from the large-leaf list), so that buckets slightly above the
+// 512-thread capacity need no further partition level.  (A 256-thread size
+// for leaves up to leaf_cap / 2, four work-groups per CU, was measured slower
+// at every leaf size.)
+template <int DT, int NTX>
+__global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
+    static_assert(NTX == 512 || NTX == 1024, "two sizes");
     using K = typename KeyOf<DT>::K;
-    constexpr int NT = kHuge ? 2 * kLeafBigWG : kLeafBigWG;
+    constexpr int NT = NTX;
     constexpr int NW = NT / 64;
-    constexpr uint32_t CAP = ((sizeof(K) == 4) ? 8192u : 4096u) * (kHuge ? 2u : 1u);  // largest leaf
-    constexpr int KPT = (int)CAP / NT;                                                // keys per thread
+    constexpr uint32_t CAP = ((sizeof(K) == 4) ? 16u : 8u) * NTX;  // largest leaf
+    constexpr int KPT = (int)CAP / NT;                             // keys per thread
     constexpr int LL = NT;  // local buckets (one per thread)
-    constexpr uint32_t kMine = kHuge ? 4u : 2u;  // the leaf kind this kernel sorts
+    constexpr uint32_t kMine = NTX == 1024 ? 4u : 2u;  // the leaf kind this kernel sorts
     // The small tables lead their LDS block so that their addresses stay
     // below 64 KiB, within reach of the immediate offset of LDS instructions.
     // P is an LDS variable of its own: the compiler then knows that plain LDS
@@ -1480,7 +1741,9 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
     // asynchronous copies write, and places no wait for the copies before
     // them (LDS atomics it waits for regardless; those here are inline asm).
     struct Lds {
-        LeafDesc sd[3];
+        LeafDesc sd[4];
+        uint32_t recs[2][8];             // raw leaf records, fetched two iterations ahead
+        uint32_t rows[2][2 * kRowsLds];  // rows of gathered leaves, staged one iteration ahead
         uint32_t hist[LL + 1];
         uint32_t bst[LL + 1];
         uint32_t s_nbig;
@@ -1505,39 +1768,71 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? min(*a.list_count, a.list_cap) : a.lv.ctl->total_buckets;
     const uint32_t G = gridDim.x;
+    // Work-groups are dispatched round-robin over the 8 XCDs (separate L2s):
+    // in every round of G leaves, the work-groups of one XCD take G/8
+    // consecutive leaves, so that neighbouring buckets, whose runs share
+    // cache lines in every tile of a tile-local level, meet in one L2.
+    const uint32_t g0 = (G % 8u == 0) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
     K* dout = (K*)a.d;
     auto big_only = [](LeafDesc ds) {
         // kinds sorted elsewhere: 1 (the wave kernel), 4 or 2 (the other
-        // instantiation); the unbounded fallback (3) belongs to the 512-thread one
-        if (ds.kind != kMine && (kHuge || ds.kind != 3u)) ds.kind = 0;
+        // instantiation); the unbounded fallback (3) and the copy of an
+        // oversize gathered bucket (5) belong to the 512-thread one
+        if (ds.kind != kMine && (NTX != 512 || (ds.kind != 3u && ds.kind != 5u))) ds.kind = 0;
         return ds;
     };
-    // Thread 0: record of g_{i+1}, in flight.  Loaded from a clamped index
-    // and discarded at decode time when past the end, so that no register
-    // of an outstanding load is overwritten (which would wait for it).
-    uint4 rec = make_uint4(0u, 0u, 0u, 0u);
-    auto fetch = [&](uint32_t idx) {
-        if (count) rec = leaf_rec(a, list_mode, min(idx, count - 1u), count);
+    // Leaf records travel to LDS asynchronously (wave 0, one dword per lane),
+    // so no register holds a record in flight.
+    auto fetch = [&](uint32_t idx, uint32_t slot) {
+        if (tid < 8 && idx < count) {
+            const uint32_t* src =
+                (list_mode ? reinterpret_cast<const uint32_t*>(a.list) : reinterpret_cast<const uint32_t*>(a.lv.bkt)) +
+                (size_t)idx * 8;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + tid),
+                                             (__attribute__((address_space(3))) void*)(S.recs[slot]), 4, 0, 0);
+        }
     };
-    auto decode = [&](uint32_t idx) {
-        return big_only(leaf_decode(a, list_mode, idx < count ? rec : make_uint4(0u, 0u, 0u, 0u)));
+    auto decode = [&](uint32_t idx, uint32_t slot) {  // thread 0, after its wave waited for the fetch
+        const uint32_t* r = S.recs[slot];
+        const LeafRaw raw{make_uint4(r[0], r[1], r[2], r[3]), make_uint4(r[4], r[5], r[6], r[7])};
+        const LeafRaw none{make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+        return big_only(leaf_decode(a, list_mode, idx < count ? raw : none));
     };
+    // the keys of leaf c into P: one contiguous range, or a bucket's runs in
+    // the tiles of a tile-local level (rows staged in rows_lds, or null)
+    auto issue_copy = [&](const LeafDesc& c, const uint32_t* rows_lds) {
+        if (c.gather)
+            gather_to_lds_async<K, NT>(P, (const K*)a.dn + c.src, a.lv.mat + c.row, c.ntiles, rows_lds, tid);
+        else
+            copy_to_lds_async<K, NT>(P, (c.from_dn ? (const K*)a.dn : (const K*)a.d) + c.start, c.n, tid);
+    };
+    // Records are fetched three leaves ahead and decoded two ahead; a
+    // gathered leaf's rows are staged in LDS one iteration before its keys
+    // are copied, so no copy waits for a global load.
     if (tid == 0) {
-        fetch(blockIdx.x);
-        sd[0] = decode(blockIdx.x);
-        fetch(blockIdx.x + G);
+        sd[0] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, g0, count)));
+        sd[1] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, g0 + G, count)));
     }
+    fetch(g0 + 2 * G, 0);
     lds_sync();
     {
         const LeafDesc c = uniform_desc(sd[0]);
-        if (c.kind == kMine) copy_to_lds_async<K, NT>(P, (c.from_dn ? (const K*)a.dn : (const K*)a.d) + c.start, c.n, tid);
+        if (c.kind == kMine) issue_copy(c, nullptr);
+        const LeafDesc c1 = uniform_desc(sd[1]);
+        if (c1.kind == kMine && c1.gather) stage_rows_async(S.rows[1], a.lv.mat + c1.row, c1.ntiles, tid);
     }
-    LeafDesc pend{0, 0, 0, 0, 0};  // sorted in A, not yet written out
-    uint32_t g = blockIdx.x;
+    LeafDesc pend{0, 0, 0, 0, 0, 0, 0, 0, 0};  // sorted in A, not yet written out
+    uint32_t g = g0;
     for (uint32_t i = 0;; ++i, g += G) {
-        vm_wait_all();  // keys of g_i in P, record of g_{i+1}, stores of g_{i-2}
-        if (tid == 0) sd[(i + 1) % 3] = decode(g + G);
+        vm_wait_all();  // keys of g_i in P, rows of g_{i+1}, record of g_{i+2}, stores of g_{i-2}
+        if (tid == 0) sd[(i + 2) % 4] = decode(g + 2 * G, i % 2);
         lds_sync();
+        {
+            // rows of g_{i+2}, staged while g_i is sorted (rows[i % 2] was
+            // read by the copy of g_i, issued in the previous iteration)
+            const LeafDesc c2 = uniform_desc(sd[(i + 2) % 4]);
+            if (c2.kind == kMine && c2.gather) stage_rows_async(S.rows[i % 2], a.lv.mat + c2.row, c2.ntiles, tid);
+        }
         // the sorted g_{i-1} leaves A after the keys of g_i have left P: a
         // read of P waits for every store issued before it
         auto write_out_pending = [&]() {
@@ -1559,8 +1854,8 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
             write_out_pending();
             break;
         }
-        const LeafDesc cur = uniform_desc(sd[i % 3]);
-        const LeafDesc nxt = uniform_desc(sd[(i + 1) % 3]);
+        const LeafDesc cur = uniform_desc(sd[i % 4]);
+        const LeafDesc nxt = uniform_desc(sd[(i + 1) % 4]);
         // 1. keys of g_i from P into registers, with their minimum and maximum
         K kv[KPT];
         if (cur.kind == kMine) {
@@ -1589,9 +1884,14 @@ __global__ __launch_bounds__(kHuge ? 2 * kLeafBigWG : kLeafBigWG, 4) void k_leaf
         if (tid < LL) hist[tid] = 0;
         if (tid == 0) s_nbig = 0;
         lds_sync();  // P consumed and A written out by every wave
-        if (nxt.kind == kMine)
-            copy_to_lds_async<K, NT>(P, (nxt.from_dn ? (const K*)a.dn : (const K*)a.d) + nxt.start, nxt.n, tid);
-        if (tid == 0) fetch(g + 2 * G);
+        if (nxt.kind == kMine) issue_copy(nxt, S.rows[(i + 1) % 2]);
+        fetch(g + 3 * G, (i + 1) % 2);  // decoded at the top of iteration i + 1
+        if (cur.kind == 5) {
+            // an oversize bucket of a tile-local level: its runs in order to
+            // d[start, start + n), where the next level partitions it again
+            gather_copy_global<K, NT>(dout + cur.start, (const K*)a.dn + cur.src, a.lv.mat + cur.row, cur.ntiles, tid);
+            continue;
+        }
         if (cur.kind == 3) {
             // Unbounded fallback: bitonic network in place in the source buffer.
             K* gsrc = (K*)((cur.from_dn ? (const K*)a.dn : (const K*)a.d) + cur.start);

@@ -35,7 +35,7 @@ GEN_UNIFORM, GEN_GAUSSIAN, GEN_SORTED, GEN_REVERSE, GEN_EQUAL, GEN_BUCKET, GEN_P
 GENERATORS = {"uniform": 0, "gaussian": 1, "sorted": 2, "reverse": 3, "equal": 4, "bucket": 5,
               "permutation": 6, "normal": 7}
 
-KERNEL_KINDS = ["splitters", "tileplan", "count", "scan", "plan", "scatter", "leaf", "init", "fill"]
+KERNEL_KINDS = ["splitters", "tileplan", "count", "scan", "plan", "scatter", "leaf", "init", "fill", "tilepart"]
 
 
 class GQSError(RuntimeError):

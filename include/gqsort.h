@@ -209,7 +209,8 @@ gqs_status gqs_set_queue_limit(gqs_ctx* ctx, uint32_t segments);
  * returns, per kernel kind, the number of launches and total milliseconds.
  * Kinds: 0 splitters, 1 tileplan, 2 count, 3 scan, 4 plan, 5 scatter, 6 leaf, 7 init, 8 fill
  * (kept for the layout of the arrays: equality buckets are filled inside the
- * leaf launches, so kind 8 reports no launches). */
+ * leaf launches, so kind 8 reports no launches), 9 tilepart (the tile-local
+ * partition of the last planned level). */
 gqs_status gqs_set_profiling(gqs_ctx* ctx, int on);
 gqs_status gqs_kernel_times(gqs_ctx* ctx, uint32_t* launches, double* ms, int nkinds);
 /* Per-launch (kind, ms) records collected by the last gqs_kernel_times();
