@@ -6,9 +6,11 @@
 // Here the whole schedule is enqueued up front: each level's bookkeeping runs
 // on the device (kernels.hip), so a sort of any size is one stream-ordered
 // chain with no host round trip.  `host_driven` restores the reference's
-// loop shape (one readback per level, stop when the queue is empty), which
-// the classic single-pivot mode uses by default because its level count is
-// data dependent (about 20 at 2^28 keys).
+// loop shape (one readback per level, stop when the queue is empty).  The
+// classic single-pivot mode's depth is data dependent (about 30 levels at
+// 2^28 keys on random input, up to n/2 on an organ pipe); its median-of-3
+// levels are followed by multipivot levels sized for the worst case, so it
+// needs no readback either.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -107,8 +109,10 @@ Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     Caps c;
     const uint64_t tile = (dtype == DT_F64) ? TileShape<uint64_t>::kTileKeys : TileShape<uint32_t>::kTileKeys;
     c.seg = n / ((uint64_t)p.leaf_max + 1) + 2;
-    const uint64_t lt = std::max<uint64_t>(1, p.leaf_target);
-    const uint64_t sum_nb = (p.mode == MODE_CLASSIC) ? 3 * c.seg : 8 * c.seg + 4 * (n / lt + 1);
+    // classic mode: single-pivot levels (3 buckets per segment), then
+    // multipivot rescue levels with the rescue leaf target
+    const uint64_t lt = std::max<uint64_t>(1, p.mode == MODE_CLASSIC ? rescue_leaf_target(p) : p.leaf_target);
+    const uint64_t sum_nb = std::max<uint64_t>(3 * c.seg, 8 * c.seg + 4 * (n / lt + 1));
     c.bkt = sum_nb + 256;
     c.mat = 257 * (n / tile + 1) + sum_nb + c.seg + 256;
     c.part = c.mat / kScanChunk + 2;
@@ -257,6 +261,32 @@ void Ctx::prof_end(hipStream_t st) {
     hipEventRecord(prof.back().b, st);
 }
 
+uint32_t rescue_leaf_target(const gqs_params& p) { return std::max<uint32_t>(1, std::min(p.leaf_target, p.leaf_max / 2)); }
+
+// Levels of a sort of n keys.  Multipivot: levels_needed (the expected depth
+// plus a spare).  Classic (one median-of-3 pivot per segment, the reference's
+// schedule): the expected classic depth (capped by max_levels), followed by
+// multipivot rescue levels sized for a segment of all n keys, so that every
+// bucket reaches the bounded local sort whatever depth the median-of-3 pivots
+// need on the input (the reference re-launches until its work list is empty,
+// CL20/QuicksortKernels.cl:132-222); all are enqueued up front and empty
+// levels exit at once.
+LevelPlan plan_levels(uint64_t n, const gqs_params& p) {
+    LevelPlan lp{levels_needed(n, p), 0, p.leaf_target};
+    if (p.mode == MODE_CLASSIC && lp.L > 0) {
+        gqs_params q = p;
+        q.mode = MODE_MULTIPIVOT;
+        q.leaf_target = rescue_leaf_target(p);
+        q.max_levels = kMaxLevels - 1;
+        const uint32_t rescue = levels_needed(n, q);
+        // without the classic spare; the rescue levels always fit
+        lp.classic = std::max<uint32_t>(1, std::min<uint32_t>(lp.L - 1, kMaxLevels - 1 - rescue));
+        lp.L = std::min<uint32_t>(lp.classic + rescue, kMaxLevels - 1);
+        lp.rescue_leaf_target = q.leaf_target;
+    }
+    return lp;
+}
+
 uint32_t levels_needed(uint64_t n, const gqs_params& p) {
     if (n <= p.leaf_max) return 0;
     uint32_t L = 0;
@@ -288,7 +318,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     const Grid g = grid(p);
     uint32_t L;
     if (offsets) {
-        L = levels_needed(n, p);
+        L = plan_levels(n, p).L;
         prof_begin(7, st);
         if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], ws.leaves, 0, 0, 0, st), "init"))
             return GQS_EDEVICE;
@@ -301,7 +331,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     } else {
         const bool as_leaf = n <= p.leaf_max;
         const bool as_huge = as_leaf && n > leaf_cap(dtype);
-        L = as_leaf ? 0 : levels_needed(n, p);
+        L = as_leaf ? 0 : plan_levels(n, p).L;
         prof_begin(7, st);
         if (fail(launch_init(ws.ctl, kCtlBlocks, keep_status, ws.segs[0], as_huge ? ws.huge : ws.leaves, 1, n,
                              SEG_RAW | (as_huge ? kInitHuge : (as_leaf ? kInitLeaf : 0u)), st),
@@ -358,6 +388,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         if (fail(launch_leaf(dtype, la, g, st), "leaf(list)")) return GQS_EDEVICE;
         prof_end(st);
     }
+    const LevelPlan lplan = plan_levels(n, p);
     stats = gqs_stats{};
     stats.levels_launched = L;
     for (uint32_t lvl = 0; lvl < L; ++lvl) {
@@ -367,10 +398,12 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         a.ctl_next = &ws.ctl[lvl + 1];
         a.is_last = (lvl + 1 == L) ? 1u : 0u;
         a.seed = 0x5EEDu + 977u * lvl;
+        a.mode = (lvl < lplan.classic) ? (uint32_t)MODE_CLASSIC : (uint32_t)MODE_MULTIPIVOT;
+        a.leaf_target = (lvl < lplan.classic) ? p.leaf_target : lplan.rescue_leaf_target;
         // The level planned to leave only leaves (the one before the spare)
         // partitions tile-locally; the spare level, contiguous, takes any
         // bucket it left above leaf_max.
-        a.tile_local = (p.mode == MODE_MULTIPIVOT && lvl + 2 == L) ? 1u : 0u;
+        a.tile_local = (a.mode == MODE_MULTIPIVOT && lvl + 2 == L) ? 1u : 0u;
         const int* steps = a.tile_local ? kLocalSteps : nullptr;
         const int nsteps = a.tile_local ? 4 : kLevelSteps;
         for (int k = 0; k < nsteps; ++k) {

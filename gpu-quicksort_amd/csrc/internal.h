@@ -85,6 +85,13 @@ gqs_status noalloc_setup(int device, unsigned char** out);
 // GQS_OK when every field of p is in the range the kernels assume.
 bool params_valid(const gqs_params& p);
 uint32_t levels_needed(uint64_t n, const gqs_params& p);
+uint32_t rescue_leaf_target(const gqs_params& p);
+struct LevelPlan {
+    uint32_t L;                   // levels enqueued
+    uint32_t classic;             // levels 0 .. classic-1 take one median-of-3 pivot (classic mode)
+    uint32_t rescue_leaf_target;  // leaf target of the multipivot levels after them
+};
+LevelPlan plan_levels(uint64_t n, const gqs_params& p);
 
 // Control block of the arrays a sort starts from (the single range, or the
 // arrays of a segmented sort that go straight to a local sort), after the

@@ -144,8 +144,7 @@ def test_golden_fixtures_through_hip(gpu_ctx):
     fx = np.load(path)
     cases = sorted({k[:-3] for k in fx.files if k.endswith("_in")})
     assert len(cases) >= 150
-    setups = [("default", {}), ("classic", dict(mode=gqsort.MODE_CLASSIC, leaf_max=1728, max_levels=40,
-                                                 host_driven=1))]
+    setups = [("default", {}), ("classic", dict(mode=gqsort.MODE_CLASSIC, leaf_max=1728, max_levels=40))]
     try:
         for name, params in setups:
             for c in cases:

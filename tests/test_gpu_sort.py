@@ -55,15 +55,71 @@ def test_float_special_values(gpu_ctx):
     check(gpu_sort(gpu_ctx, a), a, "f32 specials")
 
 
+@pytest.mark.parametrize("host_driven", [0, 1])
 @pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
-def test_classic_mode(gpu_ctx, gen):
+def test_classic_mode(gpu_ctx, gen, host_driven):
     """One median-of-3 pivot per segment and a 1728-key leaf cutoff: the
-    reference's schedule (CL12/Quicksort.h:91-97) on the same kernels."""
+    reference's schedule (CL12/Quicksort.h:91-97) on the same kernels, with
+    all levels enqueued up front (host_driven=0) or with the reference's
+    readback per level; nothing reaches the unbounded local sort."""
     a = gqsort.generate(gqsort.U32, gen, 11, 1 << 18)
-    print(f"case classic {gen}", flush=True)
-    got = gpu_sort(gpu_ctx, a, mode=gqsort.MODE_CLASSIC, leaf_max=1728, max_levels=40, host_driven=1)
-    check(got, a, f"classic/{gen}")
-    gpu_ctx.set_params(gqsort.default_params())
+    print(f"case classic {gen} host_driven={host_driven}", flush=True)
+    try:
+        got = gpu_sort(gpu_ctx, a, mode=gqsort.MODE_CLASSIC, leaf_max=1728, max_levels=40, host_driven=host_driven)
+        check(got, a, f"classic/{gen}")
+        s = gpu_ctx.stats()
+        assert s.overflow == 0 and s.oversize_leaves == 0
+    finally:
+        gpu_ctx.set_params(gqsort.default_params())
+
+
+def organ_pipe(n, dtype=np.uint32):
+    """0, 1, .., n/2-1, n/2-1, .., 1, 0: the first, middle and last keys of
+    every segment a stable single-pivot partition leaves are the segment's two
+    smallest and its largest, so the median-of-3 pivot removes two keys per
+    level (n/2 levels without a bound)."""
+    h = np.arange(n // 2, dtype=dtype)
+    return np.concatenate([h, h[::-1], np.zeros(n % 2, dtype)])
+
+
+@pytest.mark.parametrize("dt,n", [(gqsort.U32, 1 << 20), (gqsort.F64, 300001)])
+def test_classic_mode_adversarial(gpu_ctx, dt, n):
+    """Inputs that defeat median-of-3 pivots (an organ pipe and its
+    descending-then-ascending mirror) under the classic schedule with no host
+    readback: the median-of-3 levels stall, the multipivot levels enqueued
+    after them finish the sort, nothing reaches the unbounded local sort, and
+    the level count stays at what was enqueued."""
+    base = organ_pipe(n).astype(gqsort.NP_DTYPES[dt])
+    try:
+        for name, a in (("organ pipe", base), ("mirror", base.max() - base)):
+            print(f"case classic adversarial dt={dt} n={n} {name}", flush=True)
+            got = gpu_sort(gpu_ctx, a, mode=gqsort.MODE_CLASSIC, leaf_max=1728, max_levels=40, host_driven=0)
+            check(got, a, f"classic adversarial {name}")
+            s = gpu_ctx.stats()
+            assert s.overflow == 0 and s.oversize_leaves == 0
+            assert s.levels_used <= s.levels_launched <= 47
+    finally:
+        gpu_ctx.set_params(gqsort.default_params())
+
+
+def test_tile_local_rescue(gpu_ctx):
+    """Buckets of the tile-local level above leaf_max (fan-out capped at 16
+    over 100,000 keys, leaf_max 2048) are gathered into contiguous segments by
+    the leaf pass and partitioned again by the spare level: every key ends in a
+    bounded leaf.  With fan-out 2 the spare level cannot finish either and its
+    buckets take the unbounded local sort."""
+    try:
+        for fanout, leaf_max, bounded in ((16, 2048, True), (2, 64, False)):
+            for gen in ("uniform", "bucket", "sorted"):
+                a = gqsort.generate(gqsort.U32, gen, 14, 100000)
+                print(f"case tile-local rescue fanout={fanout} leaf_max={leaf_max} {gen}", flush=True)
+                got = gpu_sort(gpu_ctx, a, max_levels=2, leaf_max=leaf_max, leaf_target=32, max_fanout=fanout)
+                check(got, a, f"rescue fanout={fanout} {gen}")
+                s = gpu_ctx.stats()
+                assert s.levels_launched == 2 and s.overflow == 0
+                assert (s.oversize_leaves == 0) == bounded
+    finally:
+        gpu_ctx.set_params(gqsort.default_params())
 
 
 def test_unbounded_fallback(gpu_ctx):
