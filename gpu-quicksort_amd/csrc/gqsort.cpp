@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <thread>
 #include <atomic>
 #include <iterator>
@@ -639,15 +640,7 @@ gqs_status Ctx::sort_large(int dtype, void* d, void* dn, uint64_t n, uint64_t ru
         if (gqs_status s = run(ws, dtype, d0 + off * kb, dn0 + off * kb, m, nullptr, 0, nullptr, st, false, off > 0))
             return s;
     }
-    const uint64_t need = merge_split_entries(dtype, n);
-    if (need > merge_split_cap) {
-        hipFree(merge_split);
-        merge_split = nullptr;
-        merge_split_cap = 0;
-        if (hipMalloc(&merge_split, need * sizeof(uint64_t)) != hipSuccess)
-            return set_error(GQS_ENOMEM, "sort_large: merge scratch");
-        merge_split_cap = need;
-    }
+    if (gqs_status s = ensure_merge_split(dtype, n)) return s;
     char* src = d0;
     char* dst = dn0;
     for (uint64_t w = run_keys; w < n; w *= 2) {
@@ -668,7 +661,28 @@ gqs_status Ctx::sort_large(int dtype, void* d, void* dn, uint64_t n, uint64_t ru
     return GQS_OK;
 }
 
+gqs_status Ctx::ensure_merge_split(int dtype, uint64_t n) {
+    const uint64_t need = merge_split_entries(dtype, n);
+    if (need <= merge_split_cap) return GQS_OK;
+    hipFree(merge_split);
+    merge_split = nullptr;
+    merge_split_cap = 0;
+    if (hipMalloc(&merge_split, need * sizeof(uint64_t)) != hipSuccess)
+        return set_error(GQS_ENOMEM, "merge scratch");
+    merge_split_cap = need;
+    return GQS_OK;
+}
+
 }  // namespace gqs
+
+gqs_status gqs_set_host_pipeline(gqs_ctx* c, size_t min_keys) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (min_keys != 0 && min_keys < (1u << 16))
+        return c->set_error(GQS_EINVAL, "gqs_set_host_pipeline: min_keys must be 0 or >= 2^16");
+    c->host_pipe_min = min_keys;
+    return GQS_OK;
+}
 
 gqs_status gqs_sort_segmented(gqs_ctx* c, int dtype, void* d, void* dn, const uint64_t* offsets, size_t nseg,
                               size_t total_keys, void* stream) {
@@ -692,6 +706,11 @@ namespace gqs {
 
 constexpr size_t kStageChunk = 16ull << 20;  // bytes per pinned staging chunk
 constexpr int kHostLanes = 4;                 // copy lanes (host threads, streams)
+constexpr uint64_t kPipeMinKeys = 1ull << 24;  // default smallest host sort that overlaps its copies
+constexpr int kPipeLevels = 3;                 // merge levels of the overlapped host sort
+constexpr int kPipeRuns = 1 << kPipeLevels;    // runs uploaded and sorted in turn
+constexpr size_t kPinnedChunk = 64ull << 20;   // bytes per DMA when the host memory is pinned
+constexpr int kPipePieces = 8;                 // output ranges of the final merge, downloaded in turn
 
 void Ctx::release_lanes() {
     for (HostLane& l : lanes) {
@@ -703,15 +722,43 @@ void Ctx::release_lanes() {
         if (l.st) hipStreamDestroy(l.st);
     }
     lanes.clear();
+    if (pipe_st) {
+        hipStreamSynchronize(pipe_st);
+        hipStreamDestroy(pipe_st);
+        pipe_st = nullptr;
+    }
+    for (hipEvent_t ev : pipe_ev) hipEventDestroy(ev);
+    pipe_ev.clear();
+    for (hipEvent_t ev : chunk_ev) hipEventDestroy(ev);
+    chunk_ev.clear();
 }
 
-// Copies between pageable host memory and the device in kStageChunk pieces
-// through pinned staging buffers.  Chunks go round-robin to kHostLanes lanes;
-// a lane copies one chunk on the host while the DMA engine moves its other
-// one (host->device: fill, then copy; device->host: the next chunk's DMA is
-// issued before the current chunk is drained).  Returns when every byte has
-// arrived.
-gqs_status Ctx::host_copy(char* dev, char* host, size_t bytes, bool to_device) {
+// True if [p, p + bytes) is pinned host memory (the DMA engines reach it
+// directly).  A pageable pointer makes the query fail; its error is cleared.
+static bool host_range_pinned(const char* p, size_t bytes) {
+    for (const char* q : {p, p + bytes - 1}) {
+        hipPointerAttribute_t a{};
+        if (hipPointerGetAttributes(&a, q) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (a.type != hipMemoryTypeHost) return false;
+    }
+    return true;
+}
+
+static size_t copy_chunk(const char* host, size_t bytes) {
+    return host_range_pinned(host, bytes) ? kPinnedChunk : kStageChunk;
+}
+
+// Copies between host memory and the device in copy_chunk() pieces.  Chunks
+// go round-robin to kHostLanes lanes (one lane for pinned memory).  Pageable
+// memory goes through each lane's two pinned staging buffers: a lane copies
+// one chunk on the host while the DMA engine moves its other one
+// (host->device: fill, then copy; device->host: the next chunk's DMA is
+// issued before the current chunk is drained).  Pinned memory is copied by
+// the DMA engines directly.  Returns when every byte has arrived.
+gqs_status Ctx::host_copy(char* dev, char* host, size_t bytes, bool to_device, const CopyHooks* hooks) {
     if (lanes.empty()) {
         lanes.resize(kHostLanes);
         for (HostLane& l : lanes) {
@@ -726,8 +773,11 @@ gqs_status Ctx::host_copy(char* dev, char* host, size_t bytes, bool to_device) {
             }
         }
     }
-    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
-    const int nl = (int)std::min<size_t>(kHostLanes, nch);
+    const bool pinned = bytes && host_range_pinned(host, bytes);
+    const size_t csz = pinned ? kPinnedChunk : kStageChunk;
+    const size_t nch = (bytes + csz - 1) / csz;
+    // pinned: one lane, so chunks arrive in order at the link's full rate
+    const int nl = pinned ? 1 : (int)std::min<size_t>(kHostLanes, nch);
     std::atomic<int> bad{0};
     auto lane = [&](int li) {
         if (hipSetDevice(device) != hipSuccess) {
@@ -736,29 +786,44 @@ gqs_status Ctx::host_copy(char* dev, char* host, size_t bytes, bool to_device) {
         }
         HostLane& L = lanes[li];
         auto chunk = [&](size_t k, size_t* off, size_t* len) {
-            *off = ((size_t)li + k * (size_t)nl) * kStageChunk;
-            *len = std::min(kStageChunk, bytes - *off);
+            *off = ((size_t)li + k * (size_t)nl) * csz;
+            *len = std::min(csz, bytes - *off);
         };
         const size_t cnt = (nch - (size_t)li + nl - 1) / nl;  // chunks of this lane
         hipError_t e = hipSuccess;
         size_t off, len;
-        if (to_device) {
+        if (pinned) {
+            for (size_t k = 0; k < cnt && e == hipSuccess; ++k) {
+                chunk(k, &off, &len);
+                if (hooks && hooks->before) e = hooks->before(L.st, off, len);
+                if (e == hipSuccess)
+                    e = to_device ? hipMemcpyAsync(dev + off, host + off, len, hipMemcpyHostToDevice, L.st)
+                                  : hipMemcpyAsync(host + off, dev + off, len, hipMemcpyDeviceToHost, L.st);
+                if (e == hipSuccess && hooks && hooks->after) e = hooks->after(L.st, off, len);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(L.st);
+        } else if (to_device) {
             for (size_t k = 0; k < cnt && e == hipSuccess; ++k) {
                 const int b = (int)(k & 1);
                 chunk(k, &off, &len);
                 if (k >= 2) e = hipEventSynchronize(L.ev[b]);  // the DMA out of this buffer is done
                 if (e != hipSuccess) break;
                 std::memcpy(L.buf[b], host + off, len);
-                e = hipMemcpyAsync(dev + off, L.buf[b], len, hipMemcpyHostToDevice, L.st);
+                if (hooks && hooks->before) e = hooks->before(L.st, off, len);
+                if (e == hipSuccess) e = hipMemcpyAsync(dev + off, L.buf[b], len, hipMemcpyHostToDevice, L.st);
                 if (e == hipSuccess) e = hipEventRecord(L.ev[b], L.st);
+                if (e == hipSuccess && hooks && hooks->after) e = hooks->after(L.st, off, len);
             }
             if (e == hipSuccess) e = hipStreamSynchronize(L.st);
         } else {
             auto issue = [&](size_t k) {
                 size_t o, n;
                 chunk(k, &o, &n);
-                hipError_t r = hipMemcpyAsync(L.buf[k & 1], dev + o, n, hipMemcpyDeviceToHost, L.st);
-                return r == hipSuccess ? hipEventRecord(L.ev[k & 1], L.st) : r;
+                hipError_t r = (hooks && hooks->before) ? hooks->before(L.st, o, n) : hipSuccess;
+                if (r == hipSuccess) r = hipMemcpyAsync(L.buf[k & 1], dev + o, n, hipMemcpyDeviceToHost, L.st);
+                if (r == hipSuccess) r = hipEventRecord(L.ev[k & 1], L.st);
+                if (r == hipSuccess && hooks && hooks->after) r = hooks->after(L.st, o, n);
+                return r;
             };
             if (cnt) e = issue(0);
             for (size_t k = 0; k < cnt && e == hipSuccess; ++k) {
@@ -777,6 +842,140 @@ gqs_status Ctx::host_copy(char* dev, char* host, size_t bytes, bool to_device) {
     for (auto& t : pool) t.join();
     return bad ? set_error(GQS_EDEVICE, to_device ? "host->device staging copy" : "device->host staging copy")
                : GQS_OK;
+}
+
+// Host sort whose copies overlap the sort (2^16 <= n < kMaxRunKeys; the
+// caller holds the lock and host_d/host_dn hold n keys each).  A sort's first
+// output key depends on its last input key, so the upload and the download
+// cannot overlap each other; what overlaps them is the sorting.  The upload
+// is one continuous chunked copy on a host thread; each chunk's DMA records
+// its own event.  As soon as every chunk of run r has been issued, run r's
+// sort is enqueued on pipe_st behind those chunks' events, so the GPU sorts
+// it while later chunks are in flight.  The runs are merged in a binary tree
+// as soon as both halves of a block are sorted (level j reads d for odd j,
+// dn for even j, and writes the other), so after the last upload only the
+// last run's sort and one merge per level remain.  The final merge is
+// enqueued as kPipePieces ranges of whole merge tiles, each followed by an
+// event, and the download (one continuous chunked copy) issues each chunk's
+// DMA behind the events of the ranges it covers, so it proceeds while later
+// ranges are merged.
+gqs_status Ctx::sort_host_pipelined(int dtype, char* host, uint64_t n) {
+    const size_t kb = key_bytes(dtype);
+    const size_t bytes = n * kb;
+    char* const buf[2] = {static_cast<char*>(host_d), static_cast<char*>(host_dn)};
+    if (!pipe_st && fail(hipStreamCreateWithFlags(&pipe_st, hipStreamNonBlocking), "pipeline stream"))
+        return GQS_EDEVICE;
+    const size_t csz = copy_chunk(host, bytes);  // host_copy's chunks, for this host array
+    const size_t nch = (bytes + csz - 1) / csz;
+    for (std::vector<hipEvent_t>* pool : {&pipe_ev, &chunk_ev}) {
+        const size_t want = pool == &pipe_ev ? (size_t)kPipePieces : nch;
+        while (pool->size() < want) {
+            hipEvent_t ev = nullptr;
+            if (fail(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "pipeline event")) return GQS_EDEVICE;
+            pool->push_back(ev);
+        }
+    }
+    // runs [r*w, min((r+1)*w, n)); for n >= 2^16 the last one holds >= n/8 - 7 keys
+    const uint64_t w = (n + kPipeRuns - 1) / kPipeRuns;
+    auto edge = [&](uint64_t r) { return std::min<uint64_t>(n, r * w); };
+    if (gqs_status s = ensure_workspace(ws, w, 0, run_params(dtype, w), dtype)) return s;
+    if (gqs_status s = ensure_merge_split(dtype, n)) return s;
+    const uint64_t T = merge_tile_keys(dtype);
+    const uint64_t tiles = (n + T - 1) / T;
+    auto piece = [&](int p) { return std::min<uint64_t>(n, tiles * (uint64_t)p / kPipePieces * T); };
+
+    // upload: chunks flagged as their DMA is issued
+    struct Progress {
+        std::mutex m;
+        std::condition_variable cv;
+        std::vector<char> issued;
+        bool done = false;
+    } pg;
+    pg.issued.assign(nch, 0);
+    CopyHooks up;
+    up.after = [&](hipStream_t s, size_t off, size_t) {
+        const hipError_t e = hipEventRecord(chunk_ev[off / csz], s);
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> g(pg.m);
+            pg.issued[off / csz] = 1;
+        }
+        pg.cv.notify_all();
+        return e;
+    };
+    gqs_status up_status = GQS_OK;
+    std::thread uploader([&] {
+        up_status = host_copy(buf[0], host, bytes, true, &up);
+        {
+            std::lock_guard<std::mutex> g(pg.m);
+            pg.done = true;
+        }
+        pg.cv.notify_all();
+    });
+    // waits until chunks [c0, c1) are issued; false if the upload ended first
+    auto issued = [&](size_t c0, size_t c1) {
+        std::unique_lock<std::mutex> g(pg.m);
+        auto all = [&] {
+            for (size_t c = c0; c < c1; ++c)
+                if (!pg.issued[c]) return false;
+            return true;
+        };
+        pg.cv.wait(g, [&] { return pg.done || all(); });
+        return all();
+    };
+    // merge of blocks [lo, mid) and [mid, hi) of runs at tree level j
+    auto merge = [&](int j, uint64_t lo, uint64_t mid, uint64_t hi) {
+        const char* src = buf[(j - 1) & 1];
+        return launch_merge(dtype, src + lo * kb, mid - lo, src + mid * kb, hi - mid, buf[j & 1] + lo * kb,
+                            merge_split, pipe_st);
+    };
+    hipStream_t st = pipe_st;
+    gqs_status s = stream_enter(ws, st);
+    for (int r = 0; r < kPipeRuns && s == GQS_OK; ++r) {
+        const uint64_t off = edge(r), m = edge(r + 1) - off;
+        const size_t c0 = off * kb / csz, c1 = ((off + m) * kb + csz - 1) / csz;
+        if (!issued(c0, c1)) {
+            s = GQS_EDEVICE;  // the upload failed; its status is reported below
+            break;
+        }
+        for (size_t c = c0; c < c1 && s == GQS_OK; ++c)
+            if (fail(hipStreamWaitEvent(st, chunk_ev[c], 0), "pipeline wait")) s = GQS_EDEVICE;
+        if (s == GQS_OK) s = run(ws, dtype, buf[0] + off * kb, buf[1] + off * kb, m, nullptr, 0, nullptr, st, false, r > 0);
+        // every block of 2^j runs that run r completes, below the final level
+        for (int j = 1; j < kPipeLevels && s == GQS_OK && ((r + 1) & ((1 << j) - 1)) == 0; ++j) {
+            const uint64_t r0 = (uint64_t)r + 1 - (1ull << j);
+            if (fail(merge(j, edge(r0), edge(r0 + (1ull << (j - 1))), edge(r + 1)), "merge")) s = GQS_EDEVICE;
+        }
+    }
+    // the final level, range by range
+    const uint64_t h = edge(kPipeRuns / 2);
+    const char* fsrc = buf[(kPipeLevels - 1) & 1];
+    char* const out = buf[kPipeLevels & 1];
+    if (s == GQS_OK && fail(launch_merge_split(dtype, fsrc, h, fsrc + h * kb, n - h, merge_split, st), "merge"))
+        s = GQS_EDEVICE;
+    for (int p = 0; p < kPipePieces && s == GQS_OK; ++p) {
+        const uint64_t t0 = tiles * (uint64_t)p / kPipePieces, t1 = tiles * (uint64_t)(p + 1) / kPipePieces;
+        if (fail(launch_merge_tiles(dtype, fsrc, h, fsrc + h * kb, n - h, out, merge_split, t0, t1, st), "merge") ||
+            fail(hipEventRecord(pipe_ev[p], st), "pipeline event"))
+            s = GQS_EDEVICE;
+    }
+    const gqs_status t = stream_leave(ws, st);
+    uploader.join();
+    if (up_status != GQS_OK) return up_status;
+    if (s != GQS_OK) return s;
+    if (t != GQS_OK) return t;
+    // download: each chunk's DMA waits for the merge ranges it covers
+    CopyHooks down;
+    down.before = [&](hipStream_t ls, size_t off, size_t len) {
+        const uint64_t k0 = off / kb, k1 = (off + len) / kb;
+        for (int p = 0; p < kPipePieces; ++p)
+            if (piece(p) < k1 && piece(p + 1) > k0)
+                if (hipError_t e = hipStreamWaitEvent(ls, pipe_ev[p], 0)) return e;
+        return hipSuccess;
+    };
+    if (gqs_status e = host_copy(out, host, bytes, false, &down)) return e;
+    // the sticky status of every run decides the result
+    if (gqs_status e = fetch_stats()) return e;
+    return status_of(status_bits, "gqs_sort");
 }
 
 }  // namespace gqs
@@ -806,6 +1005,8 @@ gqs_status gqs_sort(gqs_ctx* c, int dtype, void* d, void* dn, size_t n) {
     }
     // the copy lanes' streams do not wait for earlier sorts of this context
     if (c->ws.done && c->fail(hipEventSynchronize(c->ws.done), "sync")) return GQS_EDEVICE;
+    const uint64_t pipe_min = c->host_pipe_min ? c->host_pipe_min : kPipeMinKeys;
+    if (n >= pipe_min && n < kMaxRunKeys && !c->large_run) return c->sort_host_pipelined(dtype, (char*)d, n);
     if (gqs_status s = c->host_copy((char*)c->host_d, (char*)d, bytes, true)) return s;
     if (gqs_status s = c->sort_device_locked(dtype, c->host_d, c->host_dn, n, nullptr)) return s;
     // synchronous: the sticky status of this call decides the result

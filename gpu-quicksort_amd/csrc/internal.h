@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -149,8 +150,23 @@ struct Ctx {
         hipStream_t st = nullptr;
     };
     std::vector<HostLane> lanes;
-    gqs_status host_copy(char* dev, char* host, size_t bytes, bool to_device);
+    // Per-chunk hooks of host_copy (a chunk is kStageChunk bytes at an offset
+    // that is a multiple of it): `before` runs before the chunk's DMA is
+    // issued on the lane's stream, `after` right after it is issued.
+    struct CopyHooks {
+        std::function<hipError_t(hipStream_t, size_t off, size_t len)> before, after;
+    };
+    gqs_status host_copy(char* dev, char* host, size_t bytes, bool to_device, const CopyHooks* hooks = nullptr);
     void release_lanes();
+    std::vector<hipEvent_t> chunk_ev;  // one per uploaded chunk of a pipelined host sort
+    // Host sorts of at least host_pipe_min keys (0: the default) overlap the
+    // transfers with the sort: runs sorted as they arrive, merged, and the
+    // final merge's output ranges copied back as each one is written.
+    uint64_t host_pipe_min = 0;
+    hipStream_t pipe_st = nullptr;
+    std::vector<hipEvent_t> pipe_ev;
+    gqs_status ensure_merge_split(int dtype, uint64_t n);
+    gqs_status sort_host_pipelined(int dtype, char* host, uint64_t n);
 
     bool fail(hipError_t e, const char* what);
     gqs_status set_error(gqs_status s, const char* msg);

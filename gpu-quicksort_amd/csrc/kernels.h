@@ -112,5 +112,13 @@ hipError_t launch_seed_segments(const uint64_t* offsets, uint32_t nseg_in, const
 hipError_t launch_merge(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, void* out, uint64_t* split,
                         hipStream_t st);
 uint64_t merge_split_entries(int dtype, uint64_t n);
+// The same merge in two parts: the tile boundaries once, then the output
+// tiles [tile_begin, tile_end) (tiles of merge_tile_keys(dtype) outputs), so
+// that a caller can act on each finished output range.
+hipError_t launch_merge_split(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, uint64_t* split,
+                              hipStream_t st);
+hipError_t launch_merge_tiles(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, void* out,
+                              const uint64_t* split, uint64_t tile_begin, uint64_t tile_end, hipStream_t st);
+uint64_t merge_tile_keys(int dtype);
 
 }  // namespace gqs

@@ -49,13 +49,14 @@ __global__ __launch_bounds__(kMergeWG) void k_merge_split(const typename KeyOf<D
 template <int DT>
 __global__ __launch_bounds__(kMergeWG) void k_merge_tile(const typename KeyOf<DT>::K* A, uint64_t na,
                                                         const typename KeyOf<DT>::K* B, uint64_t nb,
-                                                        typename KeyOf<DT>::K* out, const uint64_t* split) {
+                                                        typename KeyOf<DT>::K* out, const uint64_t* split,
+                                                        uint64_t tile0) {
     using K = typename KeyOf<DT>::K;
     constexpr int KPT = merge_kpt<K>();
     constexpr uint32_t T = merge_tile<K>();
     __shared__ K s[T];
     const uint32_t tid = threadIdx.x;
-    const uint64_t g = blockIdx.x;
+    const uint64_t g = tile0 + blockIdx.x;
     const uint64_t p0 = g * T, p1 = min(p0 + T, na + nb);
     // For sorted runs split[] is non-decreasing and the tile takes
     // split[g+1] - split[g] <= T keys of A.  The clamp changes nothing then;
@@ -98,14 +99,16 @@ __global__ __launch_bounds__(kMergeWG) void k_merge_tile(const typename KeyOf<DT
     for (uint32_t q = tid; q < len; q += kMergeWG) out[p0 + q] = from_key<DT>(s[q]);
 }
 
+uint64_t merge_tile_keys(int dtype) { return dtype == DT_F64 ? merge_tile<uint64_t>() : merge_tile<uint32_t>(); }
+
 uint64_t merge_split_entries(int dtype, uint64_t n) {
-    const uint64_t T = dtype == DT_F64 ? merge_tile<uint64_t>() : merge_tile<uint32_t>();
+    const uint64_t T = merge_tile_keys(dtype);
     return (n + T - 1) / T + 1;
 }
 
 template <int DT>
-static hipError_t merge_t(const void* a, uint64_t na, const void* b, uint64_t nb, void* out, uint64_t* split,
-                          hipStream_t st) {
+static hipError_t merge_split_t(const void* a, uint64_t na, const void* b, uint64_t nb, uint64_t* split,
+                                hipStream_t st) {
     using K = typename KeyOf<DT>::K;
     const uint64_t ntiles = (na + nb + merge_tile<K>() - 1) / merge_tile<K>();
     const uint64_t nsplit = ntiles + 1;
@@ -113,19 +116,46 @@ static hipError_t merge_t(const void* a, uint64_t na, const void* b, uint64_t nb
     if (ntiles > 0x7FFFFFFFull) return hipErrorInvalidValue;
     k_merge_split<DT><<<dim3((uint32_t)((nsplit + kMergeWG - 1) / kMergeWG)), dim3(kMergeWG), 0, st>>>(
         (const K*)a, na, (const K*)b, nb, split, nsplit);
-    k_merge_tile<DT><<<dim3((uint32_t)ntiles), dim3(kMergeWG), 0, st>>>((const K*)a, na, (const K*)b, nb, (K*)out,
-                                                                      split);
     return hipGetLastError();
+}
+
+template <int DT>
+static hipError_t merge_tiles_t(const void* a, uint64_t na, const void* b, uint64_t nb, void* out,
+                                const uint64_t* split, uint64_t t0, uint64_t t1, hipStream_t st) {
+    using K = typename KeyOf<DT>::K;
+    const uint64_t ntiles = (na + nb + merge_tile<K>() - 1) / merge_tile<K>();
+    t1 = t1 < ntiles ? t1 : ntiles;
+    if (t0 >= t1) return hipSuccess;
+    if (t1 - t0 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    k_merge_tile<DT><<<dim3((uint32_t)(t1 - t0)), dim3(kMergeWG), 0, st>>>((const K*)a, na, (const K*)b, nb,
+                                                                         (K*)out, split, t0);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_split(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, uint64_t* split,
+                              hipStream_t st) {
+    switch (dtype) {
+        case DT_U32: return merge_split_t<DT_U32>(a, na, b, nb, split, st);
+        case DT_F32: return merge_split_t<DT_F32>(a, na, b, nb, split, st);
+        case DT_F64: return merge_split_t<DT_F64>(a, na, b, nb, split, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_merge_tiles(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, void* out,
+                              const uint64_t* split, uint64_t tile_begin, uint64_t tile_end, hipStream_t st) {
+    switch (dtype) {
+        case DT_U32: return merge_tiles_t<DT_U32>(a, na, b, nb, out, split, tile_begin, tile_end, st);
+        case DT_F32: return merge_tiles_t<DT_F32>(a, na, b, nb, out, split, tile_begin, tile_end, st);
+        case DT_F64: return merge_tiles_t<DT_F64>(a, na, b, nb, out, split, tile_begin, tile_end, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_merge(int dtype, const void* a, uint64_t na, const void* b, uint64_t nb, void* out, uint64_t* split,
                         hipStream_t st) {
-    switch (dtype) {
-        case DT_U32: return merge_t<DT_U32>(a, na, b, nb, out, split, st);
-        case DT_F32: return merge_t<DT_F32>(a, na, b, nb, out, split, st);
-        case DT_F64: return merge_t<DT_F64>(a, na, b, nb, out, split, st);
-        default: return hipErrorInvalidValue;
-    }
+    hipError_t e = launch_merge_split(dtype, a, na, b, nb, split, st);
+    return e == hipSuccess ? launch_merge_tiles(dtype, a, na, b, nb, out, split, 0, ~0ull, st) : e;
 }
 
 }  // namespace gqs

@@ -113,6 +113,7 @@ _SIGS = {
     "gqs_sort": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     "gqs_sort_device": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
     "gqs_set_large_run": (ctypes.c_int, [_P, ctypes.c_size_t]),
+    "gqs_set_host_pipeline": (ctypes.c_int, [_P, ctypes.c_size_t]),
     "gqs_sort_device_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t, _P]),
     "gqs_sort_segmented": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t, _P]),
     "gqs_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
@@ -290,6 +291,11 @@ class Context:
 
     def sort_device(self, dtype, d, dn, n, stream=None):
         _check(self._lib.gqs_sort_device(self._h, dtype, _ptr(d), _ptr(dn), n, _stream(stream)), self)
+
+    def set_host_pipeline(self, min_keys=0):
+        """Smallest n for which gqs_sort overlaps its copies with the sort
+        (0: the default 2^24; 2**64 - 1: never)."""
+        _check(self._lib.gqs_set_host_pipeline(self._h, min_keys), self)
 
     def set_large_run(self, keys=0):
         """Sorts above `keys` keys run as sorted runs of `keys` keys merged

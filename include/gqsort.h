@@ -104,8 +104,21 @@ gqs_status gqs_default_params(int dtype, gqs_params* out);
 gqs_status gqs_check_params(const gqs_params* p);
 
 /* Reference semantics: d, dn are HOST pointers; synchronous.  The keys are
- * copied to device buffers owned by the context, sorted and copied back. */
+ * copied to device buffers owned by the context, sorted and copied back.
+ * From 2^24 keys (gqs_set_host_pipeline) the copies overlap the sort: the
+ * input goes up in eight runs, each sorted while the next ones are in
+ * flight, the runs are merged as soon as pairs are sorted, and the final
+ * merge's output goes back in eight ranges, each copied while the next is
+ * merged.  Pinned host arrays (gqs_host_alloc) are copied without staging.
+ * On an error status d's contents are unspecified. */
 gqs_status gqs_sort(gqs_ctx* ctx, int dtype, void* d, void* dn, size_t n);
+
+/* Smallest n for which gqs_sort overlaps the copies with the sort:
+ * min_keys >= 2^16, 0 for the default (2^24), SIZE_MAX to never overlap.  No
+ * counterpart in the reference (its GPUQSort wraps the host arrays as device
+ * buffers, sorts, then releases them, CL12/QuicksortMain.cpp:300-303,
+ * 381-383); tests use it to run both host paths at sizes a check can afford. */
+gqs_status gqs_set_host_pipeline(gqs_ctx* ctx, size_t min_keys);
 
 /* Device-resident: d, dn are device pointers; stream-ordered, returns after
  * enqueueing (no host synchronisation unless params.host_driven).

@@ -88,13 +88,16 @@ def test_host_pointer_sort_from_two_threads(gpu_ctx):
         check(outs[i], x, f"thread {i}")
 
 
-def test_queue_overflow_is_reported(gpu_ctx):
+@pytest.mark.parametrize("pipelined", [True, False])
+def test_queue_overflow_is_reported(gpu_ctx, pipelined):
     """A segment queue capped at one entry cannot hold level 1's child
-    segments: the host-pointer sort returns GQS_EOVERFLOW, a device sort's
-    next gqs_get_stats does, and the status clears with the next good sort."""
+    segments: the host-pointer sort (its copies overlapping the sort, or not)
+    returns GQS_EOVERFLOW, a device sort's next gqs_get_stats does, and the
+    status clears with the next good sort."""
     n = 1 << 24
     a = gqsort.generate(gqsort.U32, "uniform", 81, n)
     gpu_ctx.set_queue_limit(1)
+    gpu_ctx.set_host_pipeline(1 << 16 if pipelined else 2**64 - 1)
     try:
         d = a.copy()
         with pytest.raises(gqsort.GQSError) as ei:
@@ -110,6 +113,7 @@ def test_queue_overflow_is_reported(gpu_ctx):
         gpu_ctx.set_queue_limit(0)
     d = a.copy()
     gqsort.GPUQSort(gpu_ctx, n, d, np.empty_like(d))
+    gpu_ctx.set_host_pipeline(0)
     check(d, a, "after the cap is lifted")
     s = gpu_ctx.stats()
     assert s.overflow == 0

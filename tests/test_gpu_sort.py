@@ -150,12 +150,73 @@ def test_host_pointer_staging_chunks(gpu_ctx, dt, n):
     (and one just under a single chunk), come back bit-exact; the second call
     reuses the staging buffers."""
     gen = "uniform" if dt == gqsort.U32 else "normal"
-    for rep in range(2):
-        d = gqsort.generate(dt, gen, 30 + rep, n)
-        inp = d.copy()
-        print(f"case host staging dt={dt} n={n} rep={rep}", flush=True)
-        gqsort.GPUQSort(gpu_ctx, n, d, np.empty_like(d))
-        check(d, inp, f"host staging dt={dt} n={n}")
+    gpu_ctx.set_host_pipeline(NEVER)
+    try:
+        for rep in range(2):
+            d = gqsort.generate(dt, gen, 30 + rep, n)
+            inp = d.copy()
+            print(f"case host staging dt={dt} n={n} rep={rep}", flush=True)
+            gqsort.GPUQSort(gpu_ctx, n, d, np.empty_like(d))
+            check(d, inp, f"host staging dt={dt} n={n}")
+    finally:
+        gpu_ctx.set_host_pipeline(0)
+
+
+NEVER = 2**64 - 1
+
+
+@pytest.mark.parametrize("dt,gen,n", [(gqsort.U32, "uniform", 1 << 16), (gqsort.U32, "sorted", (1 << 16) + 1),
+                                      (gqsort.U32, "reverse", 3 * (1 << 20) + 7), (gqsort.U32, "equal", 1 << 20),
+                                      (gqsort.U32, "bucket", (1 << 26) + 11), (gqsort.F32, "normal", (5 << 20) + 3),
+                                      (gqsort.F64, "normal", (3 << 20) + 5)])
+def test_host_pointer_pipelined(gpu_ctx, dt, gen, n):
+    """gqs_sort with its copies overlapping the sort (eight runs sorted as
+    they arrive, merged in a tree, the final merge's eight output ranges
+    downloaded in turn): bit-exact from the smallest size the path takes
+    (2^16) to 2^26 keys, sizes that split unevenly into runs and merge tiles."""
+    gpu_ctx.set_host_pipeline(1 << 16)
+    try:
+        for rep in range(2):
+            d = gqsort.generate(dt, gen, 40 + rep, n)
+            inp = d.copy()
+            print(f"case host pipelined dt={dt} {gen} n={n} rep={rep}", flush=True)
+            gqsort.GPUQSort(gpu_ctx, n, d, np.empty_like(d))
+            check(d, inp, f"host pipelined dt={dt} {gen} n={n}")
+            s = gpu_ctx.stats()
+            assert s.overflow == 0
+    finally:
+        gpu_ctx.set_host_pipeline(0)
+
+
+@pytest.mark.parametrize("pipe_min", [1 << 16, NEVER])
+def test_host_pointer_pinned(gpu_ctx, pipe_min):
+    """Host arrays in pinned memory (gqs_host_alloc) are copied by the DMA
+    engines directly, with and without the overlapped path: bit-exact."""
+    import ctypes
+
+    lib = gqsort.load()
+    n = 3 * (1 << 20) + 11
+    ptrs = []
+    gpu_ctx.set_host_pipeline(pipe_min)
+    try:
+        for dt in (gqsort.U32, gqsort.F64):
+            arrs = []
+            for _ in range(2):
+                p = ctypes.c_void_p()
+                gqsort._check(lib.gqs_host_alloc(gpu_ctx.handle, n * gqsort.KEY_BYTES[dt], ctypes.byref(p)), gpu_ctx)
+                ptrs.append(p)
+                ct = ctypes.c_uint32 if dt == gqsort.U32 else ctypes.c_double
+                arrs.append(np.ctypeslib.as_array((ct * n).from_address(p.value)))
+            d, dn = arrs
+            inp = gqsort.generate(dt, "uniform" if dt == gqsort.U32 else "normal", 50, n)
+            d[:] = inp
+            print(f"case host pinned dt={dt} pipe_min={pipe_min}", flush=True)
+            gqsort.GPUQSort(gpu_ctx, n, d, dn)
+            check(d, inp, f"host pinned dt={dt}")
+    finally:
+        gpu_ctx.set_host_pipeline(0)
+        for p in ptrs:
+            lib.gqs_host_free(gpu_ctx.handle, p)
 
 
 @pytest.mark.parametrize("gen", ["uniform", "sorted", "equal", "bucket"])
