@@ -3,15 +3,24 @@
 // The reference has no serving path; its only caller sorts one array per
 // GPUQSort call (CL12/QuicksortMain.cpp:485-497).  This server admits sort
 // requests of mixed sizes and dtypes while earlier batches are still on the
-// device:
-//   * `slots` independent pipelines, each with its own HIP stream, its own
-//     sort workspace and its own device-resident key buffers (d, dn) of
-//     `slot_keys` keys, all allocated at creation inside the HBM budget;
+// device, as a three-stage pipeline:
+//   * `slots` batches in flight, each slot with its own sort workspace and
+//     device-resident key buffers (d, dn) of `slot_keys` keys, all allocated
+//     at creation inside the HBM budget;
+//   * three streams shared by the slots, one per stage (upload, sort,
+//     download): batch k+1's upload, batch k's sort and batch k-1's download
+//     run at once, so both directions of the PCIe link and the GPU are busy
+//     together.  The scheduler moves a batch to its next stage when it sees
+//     (by polling the stage's event) that the previous one has finished, so
+//     no stream waits on another stream's event: on this runtime a copy
+//     enqueued behind such a wait blocks the enqueuing thread until the wait
+//     is satisfied, which would stall every other batch's progress too;
 //   * a scheduler thread that, whenever a slot is free, packs the oldest
-//     pending requests of one dtype into a batch (up to the slot's capacity),
-//     copies them in, runs one segmented sort, copies the results out and
-//     records an event; finished batches are retired by polling their events,
-//     so new requests are admitted as soon as a slot frees up;
+//     pending requests of one dtype into a batch of about `batch_bytes`
+//     (more requests wait less behind one copy; a larger request gets a
+//     batch of its own), and retires requests one by one as their own
+//     download events fire, so a request is done when its keys are back,
+//     not when its whole batch is;
 //   * per-request timestamps (submit, admit, done) for latency statistics.
 // A request larger than a slot is rejected at submit time (GQS_EBUSY).
 #include <hip/hip_runtime.h>
@@ -34,6 +43,19 @@ using namespace gqs;
 
 namespace {
 
+constexpr uint64_t kDefaultBatchBytes = 64ull << 20;  // batch target: about 1 ms of PCIe each way
+constexpr int kUploadsAhead = 2;  // batches queued on the upload stream before another is formed
+constexpr size_t kCopyChunk = 32ull << 20;  // largest single DMA of a request's keys
+
+// A copy as DMAs of at most kCopyChunk bytes: a large request does not hold
+// a copy engine in one piece while the other direction's transfers wait.
+hipError_t chunked_copy(char* dst, const char* src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+    hipError_t e = hipSuccess;
+    for (size_t o = 0; o < bytes && e == hipSuccess; o += kCopyChunk)
+        e = hipMemcpyAsync(dst + o, src + o, std::min(kCopyChunk, bytes - o), kind, st);
+    return e;
+}
+
 double now_us() {
     using namespace std::chrono;
     return duration<double, std::micro>(steady_clock::now().time_since_epoch()).count();
@@ -49,17 +71,25 @@ struct Request {
     int state = 0;  // 0 pending, 1 running, 2 done, 3 failed
 };
 
+// A slot's batch goes through the stages in order; `stage` is the one it is
+// in (its work enqueued, its event not yet seen).
+enum Stage : int { FREE = 0, UPLOADING, SORTING, DOWNLOADING };
+
 struct Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t uploaded = nullptr;  // keys and offsets on the device (upload stream)
+    hipEvent_t sorted = nullptr;    // the sort is done (sort stream)
     Workspace ws;
     void* d = nullptr;
     void* dn = nullptr;
     uint64_t* offs_dev = nullptr;
     uint64_t* offs_host = nullptr;  // pinned
     std::vector<Request*> batch;
+    std::vector<hipEvent_t> req_done;  // per request: its keys are back (download stream), grown on demand
+    size_t retired = 0;                // requests of the batch already marked done
     size_t batch_keys = 0;
-    bool busy = false;
+    int dtype = 0;
+    uint64_t seq = 0;  // admission order
+    Stage stage = FREE;
 };
 
 }  // namespace
@@ -68,6 +98,7 @@ struct gqs_server {
     gqs_ctx* ctx = nullptr;
     gqs_server_config cfg{};
     std::vector<Slot> slots;
+    hipStream_t up = nullptr, sorter = nullptr, down = nullptr;  // the three stages, each FIFO
     std::mutex m;
     std::condition_variable cv_submit, cv_done;
     std::deque<Request*> pending;
@@ -78,41 +109,147 @@ struct gqs_server {
     gqs_server_stats stats{};
     std::string err;
 
-    void retire();
+    uint64_t next_seq = 0;
+
+    bool advance();
     bool admit();
     void loop();
+    void release_device();
+    void fail_batch(Slot& slot);
 };
 
-void gqs_server::retire() {
-    for (auto& s : slots) {
-        if (!s.busy) continue;
-        const hipError_t q = hipEventQuery(s.done);
-        if (q == hipErrorNotReady) continue;
-        const double t = now_us();
-        std::lock_guard<std::mutex> lk(m);
-        for (Request* r : s.batch) {
-            r->t_done = t;
-            r->state = (q == hipSuccess) ? 2 : 3;
-        }
-        stats.completed += s.batch.size();
-        stats.keys_sorted += s.batch_keys;
-        s.batch.clear();
-        s.batch_keys = 0;
-        s.busy = false;
-        cv_done.notify_all();
+// Waits for and frees every stream, event and buffer of the server.
+void gqs_server::release_device() {
+    for (hipStream_t st : {up, sorter, down})
+        if (st) hipStreamSynchronize(st);
+    for (auto& sl : slots) {
+        hipFree(sl.d);
+        hipFree(sl.dn);
+        hipFree(sl.offs_dev);
+        hipHostFree(sl.offs_host);
+        sl.ws.release_all();
+        for (hipEvent_t ev : {sl.uploaded, sl.sorted})
+            if (ev) hipEventDestroy(ev);
+        for (hipEvent_t ev : sl.req_done) hipEventDestroy(ev);
+        sl.req_done.clear();
+        sl.uploaded = sl.sorted = nullptr;
     }
+    for (hipStream_t st : {up, sorter, down})
+        if (st) hipStreamDestroy(st);
+    up = sorter = down = nullptr;
+}
+
+// Moves every batch whose current stage has finished on to the next one, in
+// admission order (each stage's stream is FIFO): uploaded -> enqueue its
+// sort; sorted -> enqueue its downloads, one event per request; each
+// request is done as its own download event fires, and the slot is free
+// after the last.  Returns true if anything moved.
+bool gqs_server::advance() {
+    std::vector<Slot*> order;
+    for (auto& s : slots)
+        if (s.stage != FREE) order.push_back(&s);
+    std::sort(order.begin(), order.end(), [](const Slot* a, const Slot* b) { return a->seq < b->seq; });
+    bool moved = false;
+    for (Slot* sp : order) {
+        Slot& s = *sp;
+        if (s.stage == UPLOADING) {
+            const hipError_t q = hipEventQuery(s.uploaded);
+            if (q == hipErrorNotReady) continue;
+            gqs_status st = q == hipSuccess ? GQS_OK : GQS_EDEVICE;
+            if (st == GQS_OK) {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                st = ctx->run(s.ws, s.dtype, s.d, s.dn, s.batch_keys, s.offs_dev, (uint32_t)s.batch.size(), nullptr,
+                              sorter, false);
+            }
+            if (st != GQS_OK || hipEventRecord(s.sorted, sorter) != hipSuccess) {
+                fail_batch(s);
+                continue;
+            }
+            s.stage = SORTING;
+            moved = true;
+        }
+        if (s.stage == SORTING) {
+            const hipError_t q = hipEventQuery(s.sorted);
+            if (q == hipErrorNotReady) continue;
+            const size_t kb = s.dtype == GQS_F64 ? 8 : 4;
+            hipError_t e = q;
+            for (size_t i = 0; i < s.batch.size() && e == hipSuccess; ++i) {
+                e = chunked_copy(static_cast<char*>(s.batch[i]->out),
+                                 static_cast<const char*>(s.d) + s.offs_host[i] * kb, s.batch[i]->n * kb,
+                                 hipMemcpyDeviceToHost, down);
+                if (e == hipSuccess) e = hipEventRecord(s.req_done[i], down);
+            }
+            if (e != hipSuccess) {
+                fail_batch(s);
+                continue;
+            }
+            s.stage = DOWNLOADING;
+            moved = true;
+        }
+        if (s.stage == DOWNLOADING) {
+            bool any = false;
+            while (s.retired < s.batch.size()) {
+                const hipError_t q = hipEventQuery(s.req_done[s.retired]);
+                if (q == hipErrorNotReady) break;
+                const double t = now_us();
+                std::lock_guard<std::mutex> lk(m);
+                Request* r = s.batch[s.retired++];
+                r->t_done = t;
+                r->state = (q == hipSuccess) ? 2 : 3;
+                if (q == hipSuccess) stats.completed++;
+                else stats.failed++;
+                any = true;
+            }
+            if (s.retired == s.batch.size()) {
+                std::lock_guard<std::mutex> lk(m);
+                stats.keys_sorted += s.batch_keys;
+                s.batch.clear();
+                s.retired = 0;
+                s.batch_keys = 0;
+                s.stage = FREE;
+            }
+            if (any) {
+                cv_done.notify_all();
+                moved = true;
+            }
+        }
+    }
+    return moved;
+}
+
+// Marks the requests of a batch that have not finished as failed and frees
+// its slot once the device is done with its buffers.
+void gqs_server::fail_batch(Slot& slot) {
+    for (hipStream_t st : {up, sorter, down}) hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> lk(m);
+    for (size_t i = slot.retired; i < slot.batch.size(); ++i) {
+        slot.batch[i]->state = 3;
+        slot.batch[i]->t_done = now_us();
+        stats.failed++;
+    }
+    slot.batch.clear();
+    slot.retired = 0;
+    slot.batch_keys = 0;
+    slot.stage = FREE;
+    cv_done.notify_all();
 }
 
 // Packs pending requests into the first free slot.  Returns true if a batch
-// was launched.
+// was launched.  A batch is formed only while fewer than kUploadsAhead
+// batches wait for or occupy the upload stream: the link stays busy, and
+// requests that arrive meanwhile join the next batch instead of each paying
+// a sort launch of its own.
 bool gqs_server::admit() {
     Slot* slot = nullptr;
-    for (auto& s : slots)
-        if (!s.busy) {
-            slot = &s;
-            break;
+    int uploading = 0;
+    for (auto& s : slots) {
+        if (s.stage == FREE) {
+            if (!slot) slot = &s;
+        } else if (s.stage == UPLOADING) {
+            ++uploading;
         }
-    if (!slot) return false;
+    }
+    if (!slot || uploading >= kUploadsAhead) return false;
     std::vector<Request*> batch;
     size_t keys = 0;
     int dtype = -1;
@@ -122,9 +259,12 @@ bool gqs_server::admit() {
         dtype = pending.front()->dtype;
         const size_t kb = dtype == GQS_F64 ? 8 : 4;
         const size_t cap_keys = cfg.slot_keys * 8 / kb;  // slot buffers are sized in 8-byte keys
+        // the oldest request always goes in; others while the batch stays
+        // within batch_bytes (smaller ones may pass a larger one)
+        const size_t target_keys = std::max<size_t>(1, cfg.batch_bytes / kb);
         for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch_requests;) {
             Request* r = *it;
-            if (r->dtype == dtype && keys + r->n <= cap_keys) {
+            if (r->dtype == dtype && keys + r->n <= cap_keys && (batch.empty() || keys + r->n <= target_keys)) {
                 batch.push_back(r);
                 keys += r->n;
                 it = pending.erase(it);
@@ -147,35 +287,30 @@ bool gqs_server::admit() {
     }
     slot->offs_host[batch.size()] = off;
     hipError_t e = hipSuccess;
+    while (!e && slot->req_done.size() < batch.size()) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (!e) slot->req_done.push_back(ev);
+    }
+    // upload (shared FIFO stream), in DMA pieces of at most kCopyChunk bytes
     for (size_t i = 0; i < batch.size() && !e; ++i)
-        e = hipMemcpyAsync(static_cast<char*>(slot->d) + slot->offs_host[i] * kb, batch[i]->in, batch[i]->n * kb,
-                           hipMemcpyHostToDevice, slot->stream);
+        e = chunked_copy(static_cast<char*>(slot->d) + slot->offs_host[i] * kb, static_cast<const char*>(batch[i]->in),
+                         batch[i]->n * kb, hipMemcpyHostToDevice, up);
     if (!e)
         e = hipMemcpyAsync(slot->offs_dev, slot->offs_host, (batch.size() + 1) * sizeof(uint64_t),
-                           hipMemcpyHostToDevice, slot->stream);
-    gqs_status st = GQS_OK;
-    if (!e) {
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        st = ctx->run(slot->ws, dtype, slot->d, slot->dn, off, slot->offs_dev, (uint32_t)batch.size(), nullptr,
-                      slot->stream, false);
-    }
-    for (size_t i = 0; i < batch.size() && !e && st == GQS_OK; ++i)
-        e = hipMemcpyAsync(batch[i]->out, static_cast<char*>(slot->d) + slot->offs_host[i] * kb, batch[i]->n * kb,
-                           hipMemcpyDeviceToHost, slot->stream);
-    if (!e) e = hipEventRecord(slot->done, slot->stream);
-    if (e || st != GQS_OK) {
-        std::lock_guard<std::mutex> lk(m);
-        for (Request* r : batch) {
-            r->state = 3;
-            r->t_done = now_us();
-        }
-        stats.failed += batch.size();
-        cv_done.notify_all();
+                           hipMemcpyHostToDevice, up);
+    if (!e) e = hipEventRecord(slot->uploaded, up);
+    if (e) {
+        slot->batch = std::move(batch);
+        fail_batch(*slot);
         return false;
     }
     slot->batch = std::move(batch);
+    slot->retired = 0;
     slot->batch_keys = off;
-    slot->busy = true;
+    slot->dtype = dtype;
+    slot->seq = next_seq++;
+    slot->stage = UPLOADING;
     std::lock_guard<std::mutex> lk(m);
     stats.batches++;
     stats.max_batch_requests = std::max<uint64_t>(stats.max_batch_requests, slot->batch.size());
@@ -185,12 +320,11 @@ bool gqs_server::admit() {
 void gqs_server::loop() {
     hipSetDevice(ctx->device);
     while (!stop.load()) {
-        retire();
-        bool launched = false;
-        while (admit()) launched = true;
-        if (launched) continue;
+        bool moved = advance();
+        while (admit()) moved = true;
+        if (moved) continue;
         bool any_busy = false;
-        for (auto& s : slots) any_busy |= s.busy;
+        for (auto& s : slots) any_busy |= s.stage != FREE;
         std::unique_lock<std::mutex> lk(m);
         if (stop.load()) break;
         if (any_busy) {
@@ -207,9 +341,10 @@ extern "C" {
 gqs_status gqs_server_default_config(gqs_server_config* c) {
     if (!c) return GQS_EINVAL;
     c->hbm_budget_bytes = 64ull << 30;
-    c->slots = 3;
+    c->slots = 8;  // 2 uploading + 1 sorting + the downloads queued behind them
     c->slot_keys = 0;  // derived from the budget
     c->max_batch_requests = 4096;
+    c->batch_bytes = 0;  // the default target
     return GQS_OK;
 }
 
@@ -228,6 +363,7 @@ gqs_status gqs_server_resolve_config(const gqs_server_config* in, gqs_server_con
     if (c.slot_keys == 0) c.slot_keys = c.hbm_budget_bytes / c.slots / 24;
     c.slot_keys = std::min<uint64_t>(c.slot_keys, (gqs::kMaxRunKeys - 1) / 2);
     if (c.slot_keys < 1) return GQS_EINVAL;
+    if (c.batch_bytes == 0) c.batch_bytes = kDefaultBatchBytes;
     *out = c;
     return GQS_OK;
 }
@@ -246,9 +382,12 @@ gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_serve
     s->cfg = c;
     s->slots.resize(c.slots);
     hipError_t e = hipSetDevice(device);
+    if (!e) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
+    if (!e) e = hipStreamCreateWithFlags(&s->sorter, hipStreamNonBlocking);
+    if (!e) e = hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking);
     for (auto& sl : s->slots) {
-        if (!e) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-        if (!e) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (!e) e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
+        if (!e) e = hipEventCreateWithFlags(&sl.sorted, hipEventDisableTiming);
         if (!e) e = hipMalloc(&sl.d, c.slot_keys * 8);
         if (!e) e = hipMalloc(&sl.dn, c.slot_keys * 8);
         if (!e) e = hipMalloc(reinterpret_cast<void**>(&sl.offs_dev), (c.max_batch_requests + 1) * 8);
@@ -265,15 +404,7 @@ gqs_status gqs_server_create(int device, const gqs_server_config* cfg, gqs_serve
         if (e) break;
     }
     if (e) {
-        for (auto& sl : s->slots) {
-            hipFree(sl.d);
-            hipFree(sl.dn);
-            hipFree(sl.offs_dev);
-            hipHostFree(sl.offs_host);
-            sl.ws.release_all();
-            if (sl.done) hipEventDestroy(sl.done);
-            if (sl.stream) hipStreamDestroy(sl.stream);
-        }
+        s->release_device();
         gqs_destroy(s->ctx);
         delete s;
         return GQS_ENOMEM;
@@ -381,16 +512,7 @@ gqs_status gqs_server_destroy(gqs_server* s) {
     s->cv_submit.notify_all();
     if (s->sched.joinable()) s->sched.join();
     hipSetDevice(s->ctx->device);
-    for (auto& sl : s->slots) {
-        if (sl.stream) hipStreamSynchronize(sl.stream);
-        hipFree(sl.d);
-        hipFree(sl.dn);
-        hipFree(sl.offs_dev);
-        hipHostFree(sl.offs_host);
-        sl.ws.release_all();
-        if (sl.done) hipEventDestroy(sl.done);
-        if (sl.stream) hipStreamDestroy(sl.stream);
-    }
+    s->release_device();
     {
         std::lock_guard<std::mutex> lk(s->m);
         for (auto& kv : s->reqs) {

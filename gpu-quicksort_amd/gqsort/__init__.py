@@ -57,7 +57,7 @@ class Params(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("slot_keys", ctypes.c_uint64), ("slots", ctypes.c_uint32),
-                ("max_batch_requests", ctypes.c_uint32)]
+                ("max_batch_requests", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
@@ -544,12 +544,13 @@ def device_sort(ctx, a, stream=None):
 class Server:
     """Continuous-batching sort server (gqs_server_*)."""
 
-    def __init__(self, device=0, hbm_budget_bytes=None, slots=None, slot_keys=None, max_batch_requests=None):
+    def __init__(self, device=0, hbm_budget_bytes=None, slots=None, slot_keys=None, max_batch_requests=None,
+                 batch_bytes=None):
         self._lib = load()
         cfg = ServerConfig()
         _check(self._lib.gqs_server_default_config(ctypes.byref(cfg)))
         for k, v in (("hbm_budget_bytes", hbm_budget_bytes), ("slots", slots), ("slot_keys", slot_keys),
-                     ("max_batch_requests", max_batch_requests)):
+                     ("max_batch_requests", max_batch_requests), ("batch_bytes", batch_bytes)):
             if v is not None:
                 setattr(cfg, k, v)
         h = _P()

@@ -267,21 +267,25 @@ gqs_status gqs_device_report(gqs_ctx* ctx, gqs_device_props* out);
 gqs_status gqs_generate(int dtype, int gen, uint64_t seed, void* out, size_t n);
 
 /* ---- Serving: continuous batching of sort requests ------------------------
- * A server owns a context, `slots` streams each with device-resident key
+ * A server owns a context, `slots` batch slots each with device-resident key
  * buffers of slot_keys 8-byte keys and a sort workspace (all inside the HBM
- * budget), and a scheduler thread.  Whenever a slot is free the scheduler
- * packs the oldest pending requests of one dtype (up to the slot's capacity)
- * into one segmented sort: copy in, sort, copy out.  Requests submitted while
- * other batches run are admitted as soon as a slot frees up.  in/out are host
- * buffers of n keys (pinned memory gives overlapped copies; in == out allowed);
- * they must stay valid until the request is done.  Destroy a server only when
- * no thread waits on it. */
+ * budget), one upload, one sort and one download stream, and a scheduler
+ * thread.  Whenever a slot is free the scheduler packs the oldest
+ * pending requests of one dtype (about batch_bytes, up to the slot's
+ * capacity) into one segmented sort: upload, sort, download, with the
+ * upload of one batch, the sort of the previous one and the download of the
+ * one before running at once.  A request is done when its own keys are back.
+ * in/out are host buffers of n keys (pinned memory gives overlapped copies;
+ * in == out allowed); they must stay valid until the request is done.
+ * Destroy a server only when no thread waits on it. */
 typedef struct gqs_server gqs_server;
 typedef struct gqs_server_config {
     uint64_t hbm_budget_bytes;   /* device memory the server may hold (default 64 GiB) */
     uint64_t slot_keys;          /* keys (8-byte units) per slot buffer; 0: budget / (24 * slots) */
-    uint32_t slots;              /* concurrent batches, 1..16 (default 3) */
+    uint32_t slots;              /* batches in flight, 1..16 (default 8) */
     uint32_t max_batch_requests; /* requests per batch (default 4096) */
+    uint64_t batch_bytes;        /* batch target: requests join the oldest one's batch while the
+                                    batch stays within it (0: 64 MiB) */
 } gqs_server_config;
 typedef struct gqs_server_stats {
     uint64_t submitted, completed, failed, batches, max_batch_requests, keys_sorted, pending;

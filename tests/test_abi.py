@@ -185,7 +185,8 @@ def test_server_config_caps_batches_below_one_run():
     c = gqsort.resolve_server_config(hbm_budget_bytes=1 << 30, slots=4)
     assert c.slot_keys == (1 << 30) // 4 // 24
     d = gqsort.resolve_server_config()
-    assert d.slots == 3 and d.hbm_budget_bytes == 64 << 30 and d.slot_keys == (64 << 30) // 3 // 24
+    assert d.slots == 8 and d.hbm_budget_bytes == 64 << 30 and d.slot_keys == (64 << 30) // 8 // 24
+    assert d.batch_bytes == 64 << 20 and gqsort.resolve_server_config(batch_bytes=1 << 20).batch_bytes == 1 << 20
     for kw in ({"slots": 0}, {"slots": 17}, {"max_batch_requests": 0}, {"hbm_budget_bytes": 0}):
         with pytest.raises(gqsort.GQSError):
             gqsort.resolve_server_config(**kw)
