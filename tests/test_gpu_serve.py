@@ -68,3 +68,27 @@ def test_admission_limits():
             t, out = srv.submit_array(a)
             srv.wait(t)
             assert np.array_equal(out, a)
+
+
+@pytest.mark.parametrize("slots,batch_bytes", [(1, 1 << 20), (2, 4 << 20), (8, 64 << 20)])
+def test_stage_pipeline(slots, batch_bytes):
+    """Batches move through upload, sort and download on three streams as the
+    scheduler sees each stage finish: with one slot (every stage in turn),
+    two, and the default eight; batch targets below most request sizes (each
+    larger request a batch of its own) and above; requests of all three key
+    types interleaved.  Every result equals std::sort of its request, and
+    every request is marked done only after its keys are back."""
+    reqs = make_requests(60, seed=21, lo=2**8, hi=2**21)
+    with gqsort.Server(0, hbm_budget_bytes=1 << 30, slots=slots, batch_bytes=batch_bytes) as srv:
+        print(f"case stage pipeline slots={slots} batch_bytes={batch_bytes}", flush=True)
+        pending = [srv.submit_array(a) for a in reqs]
+        for t, _ in pending:
+            srv.wait(t)
+            s, a, d = srv.times(t)
+            assert s <= a <= d
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0 and st["pending"] == 0
+        if batch_bytes <= 1 << 20:
+            assert st["batches"] >= len([a for a in reqs if a.nbytes > batch_bytes])
+    for i, (a, (_, out)) in enumerate(zip(reqs, pending)):
+        assert orc.mismatch(out, orc.std_sort(a))[0] == 0, f"request {i} (n={a.size}, dtype={a.dtype})"
