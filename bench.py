@@ -439,34 +439,57 @@ def bench_serving(args):
     }
 
 
+def organ_pipe(n):
+    """0, 1, .., n/2-1, n/2-1, .., 0: median-of-3 pivots (first, middle,
+    last) of every segment a stable partition leaves are its two smallest
+    keys and its largest, so an unbounded single-pivot schedule needs n/2
+    levels."""
+    h = np.arange(n // 2, dtype=np.uint32)
+    return np.concatenate([h, h[::-1], np.zeros(n % 2, np.uint32)])
+
+
 def bench_inputs(ctx, stream, ev0, ev1, d, dn, pristine, cases, steps):
     """Throughput of plain sorts over other generators and key types (same
     device buffers, each input restored from its pristine copy before every
     sort): Mkeys/s, fraction of the one-read-plus-one-write HBM roofline, and
-    sortedness of the last output."""
+    sortedness and multiset checksums of the last output.  A case may carry
+    parameters (the classic single-pivot schedule, the reference's)."""
     lib = gqsort.load()
     out = []
-    for dname, gen, n in cases:
+    for case in cases:
+        dname, gen, n = case[:3]
+        params = case[3] if len(case) > 3 else None
         dt = DTYPES[dname]
         kb = gqsort.KEY_BYTES[dt]
-        log(f"input {dname} {gen} n={n}")
-        host = gqsort.generate(dt, gen, 1, n)
+        log(f"input {dname} {gen} n={n} {params or ''}")
+        host = organ_pipe(n) if gen == "organ" else gqsort.generate(dt, gen, 1, n)
         pristine.upload(host, stream)
+        if params:
+            ctx.set_params(gqsort.default_params(dt), **params)
         ts = []
-        for r in range(steps + 1):
-            lib.gqs_memcpy_async(ctx.handle, d.ptr, pristine.ptr, n * kb, stream.handle)
-            ev0.record(stream)
-            ctx.sort_device(dt, d, dn, n, stream)
-            ev1.record(stream)
-            if r:
-                ts.append(ev0.elapsed_ms(ev1))
+        try:
+            for r in range(steps + 1):
+                lib.gqs_memcpy_async(ctx.handle, d.ptr, pristine.ptr, n * kb, stream.handle)
+                ev0.record(stream)
+                ctx.sort_device(dt, d, dn, n, stream)
+                ev1.record(stream)
+                if r:
+                    ts.append(ev0.elapsed_ms(ev1))
+            st = ctx.stats()
+        finally:
+            if params:
+                ctx.set_params(gqsort.default_params())
+                ctx.tune_load(gqsort.TUNED_TABLE)
         res = d.download(host.dtype, n, stream)
         sorted_ok, sums_ok = multiset_ok(host, res)
         ms = float(np.median(ts))
-        out.append({"dtype": dname, "generator": gen, "n": n, "ms": round(ms, 4),
-                    "Mkeys_s": round(n / ms / 1e3, 1),
-                    "roofline_frac": round(2 * n * kb / (ms / 1e3) / HBM_PEAK, 4), "sorted": sorted_ok,
-                    "multiset_checksums": sums_ok})
+        row = {"dtype": dname, "generator": gen, "n": n, "ms": round(ms, 4), "Mkeys_s": round(n / ms / 1e3, 1),
+               "roofline_frac": round(2 * n * kb / (ms / 1e3) / HBM_PEAK, 4), "sorted": sorted_ok,
+               "multiset_checksums": sums_ok}
+        if params:
+            row.update({"params": params, "levels_launched": st.levels_launched, "levels_used": st.levels_used,
+                        "oversize_leaves": st.oversize_leaves})
+        out.append(row)
     return out
 
 
@@ -572,6 +595,11 @@ def main():
         # (N(0,1) with outliers and duplicates): one buffer size, 1 GiB / 512 MiB
         cases = [("u32", g, n) for g in ("gaussian", "sorted", "reverse", "equal", "bucket")]
         cases += [("f32", "normal", n >> 1), ("f64", "normal", n >> 2)]
+        # the reference's schedule (one median-of-3 pivot per level, 1728-key
+        # leaves), enqueued up front, on random keys and on an organ pipe,
+        # which defeats median-of-3 pivots (its rescue levels finish it)
+        classic = {"mode": gqsort.MODE_CLASSIC, "leaf_max": 1728, "max_levels": 40, "host_driven": 0}
+        cases += [("u32", "uniform", n, classic), ("u32", "organ", n, classic), ("u32", "organ", n)]
         inputs = bench_inputs(ctx, stream, ev0, ev1, d, dn, pristine, cases, args.input_steps)
         if not all(c["sorted"] and c["multiset_checksums"] for c in inputs):
             log(f"CHECK FAILED: {inputs}")

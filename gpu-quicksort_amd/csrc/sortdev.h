@@ -1768,11 +1768,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? min(*a.list_count, a.list_cap) : a.lv.ctl->total_buckets;
     const uint32_t G = gridDim.x;
-    // Work-groups are dispatched round-robin over the 8 XCDs (separate L2s):
-    // in every round of G leaves, the work-groups of one XCD take G/8
-    // consecutive leaves, so that neighbouring buckets, whose runs share
-    // cache lines in every tile of a tile-local level, meet in one L2.
-    const uint32_t g0 = (G % 8u == 0) ? (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u : blockIdx.x;
+    const uint32_t g0 = blockIdx.x;
     K* dout = (K*)a.d;
     auto big_only = [](LeafDesc ds) {
         // kinds sorted elsewhere: 1 (the wave kernel), 4 or 2 (the other
