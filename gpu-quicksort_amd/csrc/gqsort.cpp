@@ -62,6 +62,7 @@ void Workspace::release() {
     hipFree(huge);
     hipFree(segs[0]);
     hipFree(segs[1]);
+    hipFree(gather);
     hipFree(plans);
     hipFree(spl);
     hipFree(meta);
@@ -109,20 +110,24 @@ gqs_status Ctx::stream_leave(Workspace& w, hipStream_t st) {
 Caps caps_for(uint64_t n, uint32_t nseg_in, const gqs_params& p, int dtype) {
     Caps c;
     const uint64_t tile = (dtype == DT_F64) ? TileShape<uint64_t>::kTileKeys : TileShape<uint32_t>::kTileKeys;
-    c.seg = n / ((uint64_t)p.leaf_max + 1) + 2;
+    // (at least one segment per bucket of one segment: the second of two
+    // tile-local levels takes every bucket of the first as a segment)
+    c.seg = std::max<uint64_t>(n / ((uint64_t)p.leaf_max + 1) + 2, kSplStride + 2);
     // classic mode: single-pivot levels (3 buckets per segment), then
     // multipivot rescue levels with the rescue leaf target
     const uint64_t lt = std::max<uint64_t>(1, p.mode == MODE_CLASSIC ? rescue_leaf_target(p) : p.leaf_target);
     const uint64_t sum_nb = std::max<uint64_t>(3 * c.seg, 8 * c.seg + 4 * (n / lt + 1));
     c.bkt = sum_nb + 256;
-    c.mat = 257 * (n / tile + 1) + sum_nb + c.seg + 256;
+    // (+257 * 256: the second of two tile-local levels has at most 256
+    // segments, each with a partial tile)
+    c.mat = 257 * (n / tile + 1) + sum_nb + c.seg + 256 + 257 * 256;
     c.part = c.mat / kScanChunk + 2;
     c.leaves = std::max<uint64_t>(1, nseg_in);
     // large leaves per level (or per segmented batch): each above one
     // 512-thread work-group's capacity
     c.huge = n / ((uint64_t)leaf_cap(dtype) + 1) + 2;
     // fill tasks per level: ceil(size/chunk) per equality bucket
-    c.fill = n / kFillChunk + c.bkt + 1;
+    c.fill = n / kFillChunk + c.bkt + 1 + kSplStride;  // (+: two levels' equality buckets filled at once)
     return c;
 }
 
@@ -194,11 +199,12 @@ gqs_status Ctx::ensure_caps(Workspace& ws, const Caps& need, int dtype) {
     hipError_t e = hipSuccess;
     if (!e) e = dalloc(&ws.segs[0], c.seg);
     if (!e) e = dalloc(&ws.segs[1], c.seg);
+    if (!e) e = dalloc(&ws.gather, c.seg);
     if (!e) e = dalloc(&ws.plans, c.seg);
     if (!e) e = hipMalloc(&ws.spl, std::max<size_t>(1, c.seg * kSplStride * 8));
     if (!e) e = dalloc(&ws.meta, c.seg * kSplStride);
     if (!e) e = dalloc(&ws.lut, c.seg * kLutCells);
-    if (!e) e = dalloc(&ws.mat, c.mat);
+    if (!e) e = dalloc(&ws.mat, c.mat * kMatRegions);
     if (!e) e = dalloc(&ws.bsum, c.seg * kSplStride);
     if (!e) e = dalloc(&ws.part, c.part);
     if (!e) e = dalloc(&ws.bkt, c.bkt);
@@ -367,6 +373,7 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     a.leaf_cap = leaf_cap(dtype);
     a.huge = ws.huge;
     a.huge_cap = (uint32_t)ws.cap.huge;
+    a.key_bytes = (uint32_t)key_bytes(dtype);
     // Level-0 leaves: the whole input when it fits one local sort, or the
     // short arrays of a segmented sort (all still in caller representation).
     if (offsets || L == 0) {
@@ -392,6 +399,18 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
     const LevelPlan lplan = plan_levels(n, p);
     stats = gqs_stats{};
     stats.levels_launched = L;
+    // A single range whose sort plans two partition levels and a spare runs
+    // both as tile-local levels: level 0 partitions each tile in place in d
+    // and hands every bucket to level 1 as a gathered segment (its runs in
+    // level 0's tiles); level 1 reads those runs out of d and writes its tiles
+    // to dn; the leaf pass gathers level 1's buckets from dn into d.  No count
+    // pass and no scan of per-tile counts: one read and one write of the keys
+    // per level.
+    const bool two_local = !offsets && lplan.classic == 0 && p.mode == MODE_MULTIPIVOT && L == 3 &&
+                           ws.cap.seg > (uint64_t)kSplStride && level_scheme != 1;
+    uint32_t* const mat_rows = ws.mat;
+    uint32_t* const mat_runs = ws.mat + ws.cap.mat;
+    uint32_t* const mat_next = ws.mat + 2 * ws.cap.mat;
     for (uint32_t lvl = 0; lvl < L; ++lvl) {
         a.segs = ws.segs[lvl & 1];
         a.next_segs = ws.segs[(lvl + 1) & 1];
@@ -401,18 +420,43 @@ gqs_status Ctx::run(Workspace& ws, int dtype, void* d, void* dn, uint64_t n, con
         a.seed = 0x5EEDu + 977u * lvl;
         a.mode = (lvl < lplan.classic) ? (uint32_t)MODE_CLASSIC : (uint32_t)MODE_MULTIPIVOT;
         a.leaf_target = (lvl < lplan.classic) ? p.leaf_target : lplan.rescue_leaf_target;
+        a.mat = mat_rows;
+        a.runlen = nullptr;
+        a.src_mat = nullptr;
+        a.src_pre = nullptr;
+        a.gather = nullptr;
+        a.next_gather = nullptr;
         // The level planned to leave only leaves (the one before the spare)
         // partitions tile-locally; the spare level, contiguous, takes any
         // bucket it left above leaf_max.
         a.tile_local = (a.mode == MODE_MULTIPIVOT && lvl + 2 == L) ? 1u : 0u;
         const int* steps = a.tile_local ? kLocalSteps : nullptr;
-        const int nsteps = a.tile_local ? 4 : kLevelSteps;
+        int nsteps = a.tile_local ? 4 : kLevelSteps;
+        const bool in_place = two_local && lvl == 0;
+        if (in_place) {
+            a.tile_local = 2;
+            a.runlen = mat_runs;
+            a.next_gather = ws.gather;
+            steps = kInPlaceSteps;
+            nsteps = 6;
+        } else if (two_local && lvl == 1) {
+            a.mat = mat_next;
+            a.src_mat = mat_rows;
+            a.src_pre = mat_runs;
+            a.gather = ws.gather;
+            a.run_tab = level_scheme == 2 ? 64u : 1024u;
+            steps = kGatherSteps;
+        }
         for (int k = 0; k < nsteps; ++k) {
             const int step = steps ? steps[k] : k;
+            // the in-place level scans its run lengths, not its rows
+            LevelArgs as = a;
+            if (in_place && (step == 3 || step == 4)) as.mat = mat_runs;
             prof_begin(step_kind(step), st);
-            if (fail(launch_level_step(step, dtype, a, g, st), "level")) return GQS_EDEVICE;
+            if (fail(launch_level_step(step, dtype, as, g, st), "level")) return GQS_EDEVICE;
             prof_end(st);
         }
+        if (in_place) continue;  // no leaves: every bucket is a segment of level 1
         LeafArgs la{};
         la.d = d;
         la.dn = dn;
@@ -612,6 +656,14 @@ gqs_status gqs_set_queue_limit(gqs_ctx* c, uint32_t segments) {
     if (!c) return GQS_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
     c->queue_limit = segments;
+    return GQS_OK;
+}
+
+gqs_status gqs_set_level_scheme(gqs_ctx* c, int scheme) {
+    if (!c) return GQS_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (scheme < 0 || scheme > 2) return c->set_error(GQS_EINVAL, "gqs_set_level_scheme: scheme must be 0, 1 or 2");
+    c->level_scheme = (uint32_t)scheme;
     return GQS_OK;
 }
 
@@ -1059,6 +1111,7 @@ gqs_status Ctx::partition_level(int dtype, void* d, void* dn, uint64_t n, LevelR
     a.huge = ws.huge;
     a.huge_cap = (uint32_t)ws.cap.huge;
     a.status = &ws.ctl[kSeedCtl].overflow;
+    a.key_bytes = (uint32_t)key_bytes(dtype);
     const Grid g = grid(p);
     for (int step = 0; step < kLevelSteps; ++step)
         if (fail(launch_level_step(step, dtype, a, g, st), "level")) return GQS_EDEVICE;

@@ -25,6 +25,7 @@ inline uint32_t leaf_cap(int dtype) { return dtype == DT_F64 ? 4096u : 8192u; }
 // Device workspace, grown on demand and reused across sorts.
 struct Workspace {
     SegRec* segs[2] = {nullptr, nullptr};  // level queues (ping-pong)
+    SegGather* gather = nullptr;           // runs of the gathered segments of a second tile-local level
     SegPlan* plans = nullptr;
     void* spl = nullptr;
     uint16_t* meta = nullptr;
@@ -56,15 +57,25 @@ struct ProfRec {
 
 // Steps of one partition level and their profiling kinds.  Step ids: 0
 // splitters, 1 tileplan, 2 count, 3 scan_reduce, 4 scan_apply, 5 plan, 6
-// scatter, 7 tilepart, 8 plan of a tile-local level.  A contiguous level runs
-// 0-6, a tile-local level 0, 1, 7, 8 (equality buckets are filled by the
-// level's leaf pass either way).
+// scatter, 7 tilepart, 8 plan of a tile-local level, 9 plan of an in-place
+// tile-local level, 10 tilepart of gathered segments.  A contiguous level
+// runs 0-6, a tile-local level 0, 1, 7, 8 (equality buckets are filled by the
+// level's leaf pass either way); a sort of two tile-local levels runs 0, 1,
+// 7, 3, 4, 9 (in place, the scan over its run lengths, no leaf pass) and then
+// 0, 1, 10, 8.
 constexpr int kLevelSteps = 7;
 constexpr int kLocalSteps[4] = {0, 1, 7, 8};
+constexpr int kInPlaceSteps[6] = {0, 1, 7, 3, 4, 9};
+constexpr int kGatherSteps[4] = {0, 1, 10, 8};
 inline int step_kind(int step) {
-    static const int kinds[9] = {0, 1, 2, 3, 3, 4, 5, 9, 4};
+    static const int kinds[11] = {0, 1, 2, 3, 3, 4, 5, 9, 4, 4, 9};
     return kinds[step];
 }
+// The count matrix holds three regions of Caps::mat entries: a level's own
+// rows (region 0), and for a sort of two tile-local levels the first level's
+// scanned run lengths (region 1) and the second level's rows (region 2),
+// while the first level's rows stay in region 0.
+constexpr int kMatRegions = 3;
 hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid& g, hipStream_t st);
 
 // One run of the partition levels holds fewer than kMaxRunKeys keys: segment
@@ -134,6 +145,7 @@ struct Ctx {
     gqs_stats stats{};
     uint32_t status_bits = 0;  // sticky overflow word of the last call (after fetch_stats)
     uint32_t queue_limit = 0;  // testing hook: caps the segment queue (0: none)
+    uint32_t level_scheme = 0;  // testing hook (gqs_set_level_scheme): 0 default, 1 contiguous + tile-local, 2 short run tables
     unsigned char* noalloc_scratch = nullptr;  // this device's allocation-free workspace (noalloc.hip)
     bool profiling = false;
     std::vector<ProfRec> prof;

@@ -44,9 +44,19 @@ struct LevelArgs {
     uint32_t leaf_cap;       // largest leaf of one 512-thread work-group (keys)
     LeafRec* huge;           // leaves above leaf_cap (and <= leaf_max), appended by k_plan
     uint32_t huge_cap;       // capacity of huge
-    uint32_t tile_local;     // 1: this level partitions each tile in place into dn (k_tilepart);
-                             // its buckets are gathered leaves, those above leaf_max rescue segments
+    uint32_t tile_local;     // 1: this level partitions each tile into the same range of dn
+                             // (k_tilepart); its buckets are gathered leaves, those above
+                             // leaf_max rescue segments.  2: each tile in place in d, with the
+                             // run lengths in runlen; every bucket becomes a gathered segment
+                             // of the next level (k_plan_gather)
     uint32_t* status;        // the call's sticky overflow word (every bound hit is also ORed here)
+    SegGather* next_gather;  // tile_local 2: where each next-level segment's runs are (next_segs' slots)
+    const SegGather* gather; // gathered segments (SEG_GATHER): where this level's segments' runs are
+    uint32_t* runlen;        // tile_local 2: keys of bucket b in tile t, laid out as mat (then scanned)
+    const uint32_t* src_mat; // gathered segments: the partitioning level's in-tile starts S(b, t)
+    const uint32_t* src_pre; // gathered segments: its scanned run lengths (P(b, t) by difference)
+    uint32_t key_bytes;      // 4 or 8
+    uint32_t run_tab;        // gathered segments: run-table entries per pass (kScatterWG; fewer in tests)
 };
 
 // Local-sort (leaf) arguments.  Either an explicit list of segments

@@ -281,6 +281,12 @@ __global__ __launch_bounds__(256) void k_scan_apply(LevelArgs a) {
     }
 }
 
+// Splitter idx of the level's splitter table, widened to 64 bits.
+__device__ __forceinline__ uint64_t spl_key(const LevelArgs& a, uint32_t idx) {
+    return a.key_bytes == 8 ? reinterpret_cast<const uint64_t*>(a.spl)[idx]
+                            : (uint64_t)reinterpret_cast<const uint32_t*>(a.spl)[idx];
+}
+
 // ------------------------------------------------------------ k_plan
 // One work-group per segment, one thread per bucket (nbuckets <= 256).
 __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
@@ -333,12 +339,12 @@ __global__ __launch_bounds__(256) void k_plan(LevelArgs a) {
         if (threadIdx.x == 0 && ftotal) fbase = atomicAdd(&a.ctl->nfill, ftotal);
         __syncthreads();
         if (nfill) {
-            const uint32_t spl_index = (uint32_t)s * kSplStride + (eqi[b] - 1u);
+            const uint64_t key = spl_key(a, (uint32_t)s * kSplStride + (eqi[b] - 1u));
             for (uint32_t c = 0; c < nfill; ++c) {
                 const uint32_t slot = fbase + fex + c;
                 const uint32_t off = c * kFillChunk;
                 if (slot < a.fill_cap)
-                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), spl_index};
+                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), 0u, key};
                 else
                     flag_overflow(a.ctl, a.status, kOvfFill);
             }
@@ -408,14 +414,78 @@ __global__ __launch_bounds__(256) void k_plan_local(LevelArgs a) {
         if (threadIdx.x == 0 && ftotal) fbase = atomicAdd(&a.ctl->nfill, ftotal);
         __syncthreads();
         if (nfill) {
-            const uint32_t spl_index = (uint32_t)s * kSplStride + (eqi[b] - 1u);
+            const uint64_t key = spl_key(a, (uint32_t)s * kSplStride + (eqi[b] - 1u));
             for (uint32_t c = 0; c < nfill; ++c) {
                 const uint32_t slot = fbase + fex + c;
                 const uint32_t off = c * kFillChunk;
                 if (slot < a.fill_cap)
-                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), spl_index};
+                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), 0u, key};
                 else
                     flag_overflow(a.ctl, a.status, kOvfFill);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ k_plan_gather
+// Plan of an in-place tile-local level (tile_local 2, after k_tilepart and
+// the scan of its run lengths), one work-group per segment, one thread per
+// bucket: every non-empty bucket becomes a gathered segment of the next level
+// (SEG_GATHER: its runs in this level's tiles of d, its final place in the
+// sorted segment), and an equality bucket becomes fill tasks that the next
+// level's leaf pass runs, once that level has read its runs out of d.  The
+// fill tasks carry their value: the next level's splitters overwrite this
+// level's table.
+__global__ __launch_bounds__(256) void k_plan_gather(LevelArgs a) {
+    __shared__ uint16_t eqi[kSplStride + 1];  // splitter index + 1 of an equality bucket, else 0
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t fbase;
+    __shared__ uint32_t rsum[kSplStride + 1];  // bucket starts in the segment, then its size
+    const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
+    for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const SegRec seg = a.segs[s];
+        const SegPlan p = a.plans[s];
+        for (uint32_t i = threadIdx.x; i <= (uint32_t)kSplStride; i += 256) eqi[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kSplStride; i += 256) {
+            const uint16_t m = a.meta[(size_t)s * kSplStride + i];
+            if (m >> 8) eqi[(m & 0xFFu) + 1] = (uint16_t)(i + 1);
+        }
+        const uint32_t b = threadIdx.x;
+        const uint32_t bs = (b < p.nbuckets) ? a.bsum[(size_t)s * kSplStride + b] : 0u;
+        uint32_t stot;
+        rsum[b] = block_excl_scan<256>(bs, wsum, &stot);
+        if (b == 0) rsum[p.nbuckets] = stot;
+        __syncthreads();
+        uint32_t rel = 0, size = 0, nfill = 0;
+        if (b < p.nbuckets) {
+            rel = rsum[b];
+            size = rsum[b + 1] - rel;
+            if (eqi[b] != 0 && size > 0) {
+                nfill = (size + kFillChunk - 1) / kFillChunk;
+            } else if (size > 0) {
+                const uint32_t slot = atomicAdd(&a.ctl_next->nseg, 1u);
+                if (slot < a.seg_cap) {
+                    a.next_segs[slot] = SegRec{seg.start + rel, size, SEG_GATHER};
+                    a.next_gather[slot] = SegGather{seg.start, p.mat_base + b * p.ntiles, p.ntiles};
+                } else
+                    flag_overflow(a.ctl_next, a.status, kOvfSegs);
+            }
+        }
+        uint32_t ftotal;
+        const uint32_t fex = block_excl_scan<256>(nfill, wsum, &ftotal);
+        if (threadIdx.x == 0 && ftotal) fbase = atomicAdd(&a.ctl_next->nfill, ftotal);
+        __syncthreads();
+        if (nfill) {
+            const uint64_t key = spl_key(a, (uint32_t)s * kSplStride + (eqi[b] - 1u));
+            for (uint32_t c = 0; c < nfill; ++c) {
+                const uint32_t slot = fbase + fex + c;
+                const uint32_t off = c * kFillChunk;
+                if (slot < a.fill_cap)
+                    a.fill[slot] = FillTask{seg.start + rel + off, min(kFillChunk, size - off), 0u, key};
+                else
+                    flag_overflow(a.ctl_next, a.status, kOvfFill);
             }
         }
         __syncthreads();
@@ -446,7 +516,8 @@ hipError_t launch_level_step(int step, int dtype, const LevelArgs& a, const Grid
         case 4: hipLaunchKernelGGL(k_scan_apply, dim3(g.scan), dim3(256), 0, st, a); return hipGetLastError();
         case 5: hipLaunchKernelGGL(k_plan, dim3(g.plan), dim3(256), 0, st, a); return hipGetLastError();
         case 8: hipLaunchKernelGGL(k_plan_local, dim3(g.plan), dim3(256), 0, st, a); return hipGetLastError();
-        case 0: case 2: case 6: case 7: break;  // typed: splitters, count, scatter, tilepart
+        case 9: hipLaunchKernelGGL(k_plan_gather, dim3(g.plan), dim3(256), 0, st, a); return hipGetLastError();
+        case 0: case 2: case 6: case 7: case 10: break;  // typed: splitters, count, scatter, tilepart (2 kinds)
         default: return hipErrorInvalidValue;
     }
     switch (dtype) {

@@ -18,7 +18,8 @@ hipError_t typed_level_step(int step, const LevelArgs& a, const Grid& g, hipStre
             hipLaunchKernelGGL(k_scatter<DT>, dim3(std::max(1, g.part * kPartWG / kScatterWG)), dim3(kScatterWG), 0, st,
                                a);
             break;
-        case 7: hipLaunchKernelGGL(k_tilepart<DT>, dim3(g.tilepart), dim3(kScatterWG), 0, st, a); break;
+        case 7: hipLaunchKernelGGL((k_tilepart<DT, false>), dim3(g.tilepart), dim3(kScatterWG), 0, st, a); break;
+        case 10: hipLaunchKernelGGL((k_tilepart<DT, true>), dim3(g.tilepart), dim3(kScatterWG), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

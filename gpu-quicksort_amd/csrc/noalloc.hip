@@ -38,13 +38,14 @@ size_t carve(Workspace& w, unsigned char* base, const Caps& c) {
     };
     take(&w.segs[0], c.seg, sizeof(SegRec));
     take(&w.segs[1], c.seg, sizeof(SegRec));
+    take(&w.gather, c.seg, sizeof(SegGather));
     take(&w.plans, c.seg, sizeof(SegPlan));
     unsigned char* spl = nullptr;
     take(&spl, c.seg * kSplStride, 8);
     w.spl = spl;
     take(&w.meta, c.seg * kSplStride, sizeof(uint16_t));
     take(&w.lut, c.seg * kLutCells, sizeof(uint8_t));
-    take(&w.mat, c.mat, sizeof(uint32_t));
+    take(&w.mat, c.mat * kMatRegions, sizeof(uint32_t));
     take(&w.bsum, c.seg * kSplStride, sizeof(uint32_t));
     take(&w.part, c.part, sizeof(uint32_t));
     take(&w.bkt, c.bkt, sizeof(LeafRec));

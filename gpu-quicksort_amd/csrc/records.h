@@ -29,13 +29,28 @@ enum Mode : int { MODE_MULTIPIVOT = 0, MODE_CLASSIC = 1 };
 // SegRec::flags
 constexpr uint32_t SEG_SRC_DN = 1u;  // keys live in dn (else in d)
 constexpr uint32_t SEG_RAW = 2u;     // keys are still in caller representation
-constexpr uint32_t SEG_GATHER = 4u;  // a leaf whose keys are runs of a tile-local level's tiles
+constexpr uint32_t SEG_GATHER = 4u;  // keys are runs of a tile-local level's tiles (a leaf or a segment)
 
-// A key range [start, start+n) to partition (or to sort locally).
+// A key range to partition: [start, start+n) of d (or of dn with SEG_SRC_DN).
+// With SEG_GATHER the segment's keys are instead the n keys of one bucket of
+// a segment that an in-place tile-local level partitioned (their runs are
+// described by the SegGather entry of the same queue slot); `start` is then
+// where they go (the next level writes its tiles to dn from there).
 struct SegRec {
     uint64_t start;
     uint32_t n;
     uint32_t flags;
+};
+
+// Where the keys of a gathered segment are: bucket b of a segment whose tile
+// t (kTileKeys keys of d from src + t * kTileKeys) holds the bucket's keys at
+// [S(b, t), S(b + 1, t)), S(b, .) being the row src_mat[row ...] of ntiles
+// in-tile starts and S(b + 1, .) the row after it; the run of tile t sits at
+// P(b, t) = src_pre[row + t] - src_pre[row] in the segment's key order.
+struct SegGather {
+    uint64_t src;     // first key of the partitioned segment's tiles in d
+    uint32_t row;     // count-matrix index of S(b, 0)
+    uint32_t ntiles;  // tiles of the partitioned segment
 };
 
 // A leaf to sort locally: an entry of the level's bucket table, of the list
@@ -95,7 +110,8 @@ struct LevelCtl {
 struct FillTask {
     uint64_t start;
     uint32_t n;
-    uint32_t spl_index;  // index of the key value in the splitter table
+    uint32_t pad;
+    uint64_t key;  // the value, in key representation
 };
 constexpr uint32_t kFillChunk = 65536;
 

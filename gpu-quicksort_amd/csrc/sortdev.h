@@ -167,68 +167,6 @@ __device__ __forceinline__ bool wave_one_bucket(const uint32_t (&b)[N], const bo
     return __ballot(!same) == 0;
 }
 
-// hist_rank for N keys per thread at once: every key's atomic is issued
-// before any result is waited for.  Invalid keys (valid[j] false) take no
-// slot and get rank 0.
-template <int N>
-__device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[N], const bool (&valid)[N],
-                                            uint32_t (&r)[N]) {
-    if (!clustered(b[0], valid[0])) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            r[j] = 0;
-            if (valid[j]) r[j] = atomicAdd(&hist[b[j]], 1u);
-        }
-        return;
-    }
-    {
-        // the whole wave in one bucket: one add for all its keys, ranks in
-        // slot-major, lane-minor order
-        uint32_t b0, l0;
-        if (wave_one_bucket<N>(b, valid, b0, l0)) {
-            uint64_t vm[N];
-            uint32_t before[N], total = 0;
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-                vm[j] = __ballot(valid[j]);
-                before[j] = total;
-                total += (uint32_t)__popcll(vm[j]);
-            }
-            uint32_t base = 0;
-            if (lane_id() == l0) base = atomicAdd(&hist[b0], total);
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, l0);
-#pragma unroll
-            for (int j = 0; j < N; ++j) r[j] = valid[j] ? base + before[j] + mbcnt(vm[j]) : 0u;
-            return;
-        }
-    }
-    // as hist_add2: a wave of at most two buckets takes one slot range per
-    // bucket (rank = range base + lanes below in the same bucket).  All
-    // atomics are issued first; the second pass recomputes the (wave-uniform)
-    // peel instead of keeping it, and reads the bases from the leader lanes.
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const Peel pl = peel2(b[j], valid[j]);
-        r[j] = 0;
-        if (pl.two) {
-            if (lane == pl.l0) r[j] = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m0));
-            if (pl.m1 && lane == pl.l1) r[j] = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m1));
-        } else if (valid[j]) {
-            r[j] = atomicAdd(&hist[b[j]], 1u);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const Peel pl = peel2(b[j], valid[j]);
-        if (pl.two) {
-            const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)r[j], pl.l0);
-            const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)r[j], pl.l1);
-            r[j] = ((pl.m0 >> lane) & 1ull) ? r0 + mbcnt(pl.m0) : r1 + mbcnt(pl.m1);
-        }
-    }
-}
-
 // LDS atomics written as inline assembly, for kernels with global->LDS copies
 // in flight.  The compiler makes every LDS atomic it sees wait for all
 // outstanding global->LDS copies (it cannot tell which LDS they write); these
@@ -270,6 +208,111 @@ __device__ __forceinline__ void hist_add_asm(uint32_t* hist, uint32_t b) {
         lds_add(&hist[b], 1u);
     }
 }
+// Eight returning LDS adds of per-lane values in one asm statement, waited
+// for together (as lds_add_rtn4).
+__device__ __forceinline__ void lds_add_rtn8v(const uint32_t (&ad)[8], const uint32_t (&v)[8], uint32_t (&r)[8]) {
+    asm volatile(
+        "ds_add_rtn_u32 %0, %8, %16\n\t"
+        "ds_add_rtn_u32 %1, %9, %17\n\t"
+        "ds_add_rtn_u32 %2, %10, %18\n\t"
+        "ds_add_rtn_u32 %3, %11, %19\n\t"
+        "ds_add_rtn_u32 %4, %12, %20\n\t"
+        "ds_add_rtn_u32 %5, %13, %21\n\t"
+        "ds_add_rtn_u32 %6, %14, %22\n\t"
+        "ds_add_rtn_u32 %7, %15, %23\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]), "=&v"(r[7])
+        : "v"(ad[0]), "v"(ad[1]), "v"(ad[2]), "v"(ad[3]), "v"(ad[4]), "v"(ad[5]), "v"(ad[6]), "v"(ad[7]),
+          "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(v[4]), "v"(v[5]), "v"(v[6]), "v"(v[7])
+        : "memory");
+}
+
+// hist_rank for N keys per thread at once: every key's atomic is issued
+// before any result is waited for.  Invalid keys (valid[j] false) take no
+// slot and get rank 0.
+// With kAsm the adds are inline assembly (lds_add_rtn*), for kernels that
+// have global->LDS copies in flight.
+template <int N, bool kAsm = false>
+__device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[N], const bool (&valid)[N],
+                                            uint32_t (&r)[N]) {
+    // one add; the returning asm form waits for its own result
+    auto add = [&](uint32_t* p, uint32_t v) -> uint32_t {
+        if constexpr (kAsm) return lds_add_rtn(p, v);
+        else return atomicAdd(p, v);
+    };
+    if (!clustered(b[0], valid[0])) {
+        if constexpr (kAsm) {
+            static_assert(N % 8 == 0, "adds go in eights");
+            // keys of invalid slots add 0 (to bucket 0): no effect
+#pragma unroll
+            for (int j = 0; j < N; j += 8) {
+                uint32_t ad[8], v[8], rr[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    ad[u] = lds_addr(&hist[valid[j + u] ? b[j + u] : 0u]);
+                    v[u] = valid[j + u] ? 1u : 0u;
+                }
+                lds_add_rtn8v(ad, v, rr);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) r[j + u] = valid[j + u] ? rr[u] : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                r[j] = 0;
+                if (valid[j]) r[j] = atomicAdd(&hist[b[j]], 1u);
+            }
+        }
+        return;
+    }
+    {
+        // the whole wave in one bucket: one add for all its keys, ranks in
+        // slot-major, lane-minor order
+        uint32_t b0, l0;
+        if (wave_one_bucket<N>(b, valid, b0, l0)) {
+            uint64_t vm[N];
+            uint32_t before[N], total = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                vm[j] = __ballot(valid[j]);
+                before[j] = total;
+                total += (uint32_t)__popcll(vm[j]);
+            }
+            uint32_t base = 0;
+            if (lane_id() == l0) base = add(&hist[b0], total);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, l0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) r[j] = valid[j] ? base + before[j] + mbcnt(vm[j]) : 0u;
+            return;
+        }
+    }
+    // as hist_add2: a wave of at most two buckets takes one slot range per
+    // bucket (rank = range base + lanes below in the same bucket).  All
+    // atomics are issued first; the second pass recomputes the (wave-uniform)
+    // peel instead of keeping it, and reads the bases from the leader lanes.
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const Peel pl = peel2(b[j], valid[j]);
+        r[j] = 0;
+        if (pl.two) {
+            if (lane == pl.l0) r[j] = add(&hist[b[j]], (uint32_t)__popcll(pl.m0));
+            if (pl.m1 && lane == pl.l1) r[j] = add(&hist[b[j]], (uint32_t)__popcll(pl.m1));
+        } else if (valid[j]) {
+            r[j] = add(&hist[b[j]], 1u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const Peel pl = peel2(b[j], valid[j]);
+        if (pl.two) {
+            const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)r[j], pl.l0);
+            const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)r[j], pl.l1);
+            r[j] = ((pl.m0 >> lane) & 1ull) ? r0 + mbcnt(pl.m0) : r1 + mbcnt(pl.m1);
+        }
+    }
+}
+
 // Same, returning each lane's rank among the tile's keys of its bucket.
 
 // Splitter classification: i = #{splitters < key} by a branch-free search over
@@ -484,6 +527,23 @@ __device__ void global_bitonic(K* buf, uint64_t n, uint32_t tid, uint32_t nthrea
 }
 
 // ------------------------------------------------------------ k_splitters
+// Gathered segments (SEG_GATHER): the index in d of the key at position pos
+// of the segment.  The run of tile t starts at P(t) = pre[t] - pre[0] in the
+// segment's key order; the key is in the last tile t with P(t) <= pos (that
+// run is not empty: P(t + 1) > pos, or t is the last tile).
+template <class K>
+__device__ __forceinline__ uint64_t gathered_index(const LevelArgs& a, const SegGather& g, uint32_t pos) {
+    const uint32_t* pre = a.src_pre + g.row;
+    const uint32_t base = pre[0];
+    uint32_t lo = 0, hi = g.ntiles;  // the tile is in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] - base <= pos) lo = mid;
+        else hi = mid;
+    }
+    return g.src + (uint64_t)lo * TileShape<K>::kTileKeys + a.src_mat[g.row + lo] + (pos - (pre[lo] - base));
+}
+
 template <int DT>
 __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
@@ -498,6 +558,13 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         const bool raw = seg.flags & SEG_RAW;
         const uint32_t n = seg.n;
+        // key at position pos of the segment (a gathered segment: through its runs)
+        const bool gathered = seg.flags & SEG_GATHER;
+        const SegGather gs = gathered ? a.gather[s] : SegGather{0, 0, 0};
+        auto key_at = [&](uint32_t pos) -> K {
+            if (gathered) return src[gathered_index<K>(a, gs, pos)];
+            return load_key<DT>(src, seg.start + pos, raw);
+        };
         K* spl = (K*)a.spl + (size_t)s * kSplStride;
         uint16_t* meta = a.meta + (size_t)s * kSplStride;
         if (a.tile_local)
@@ -506,9 +573,9 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
         uint32_t ncand;
         if (a.mode == MODE_CLASSIC) {
             if (tid == 0) {
-                const K x1 = load_key<DT>(src, seg.start, raw);
-                const K x2 = load_key<DT>(src, seg.start + (n >> 1), raw);
-                const K x3 = load_key<DT>(src, seg.start + n - 1, raw);
+                const K x1 = key_at(0);
+                const K x2 = key_at(n >> 1);
+                const K x3 = key_at(n - 1);
                 // median-of-3, CL12/Quicksort.h:59-73
                 K m;
                 if (x1 < x2) m = (x2 < x3) ? x2 : ((x1 < x3) ? x3 : x1);
@@ -535,7 +602,7 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
                 if (j < S) {
                     const uint64_t r = hash32(a.seed, (uint32_t)seg.start ^ (uint32_t)(seg.start >> 32), j) % n;
                     const uint64_t pos = ((uint64_t)j * n + r) / S;
-                    sv[e] = load_key<DT>(src, seg.start + pos, raw);
+                    sv[e] = key_at((uint32_t)pos);
                 }
             }
 #pragma unroll
@@ -1078,19 +1145,219 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
     }
 }
 
+// ------------------------------------------------------------ gathered tiles
+// A level whose segments are gathered (SEG_GATHER: buckets of the in-place
+// tile-local level before it) reads each of its tiles, kTileKeys consecutive
+// keys of the segment's key order, from the runs that bucket left in the
+// partitioned tiles: run t is S(b, t) .. S(b + 1, t) of tile t and sits at
+// P(t) = pre[t] - pre[0] in the segment's key order.
+
+// The last tile t of gathered segment g with P(t) <= pos, found by one wave
+// (64 probes per round); every lane returns it.
+__device__ __forceinline__ uint32_t run_find(const LevelArgs& a, const SegGather& g, uint32_t pos) {
+    const uint32_t lane = lane_id();
+    const uint32_t* pre = a.src_pre + g.row;
+    const uint32_t base = pre[0];
+    uint32_t lo = 0, len = g.ntiles;  // the tile is in [lo, lo + len); P(lo) <= pos
+    while (len > 1) {
+        const uint32_t step = (len + 63) >> 6;
+        const bool ok = lane * step < len && pre[lo + lane * step] - base <= pos;
+        const uint32_t l = 63u - (uint32_t)__clzll(__ballot(ok));  // lane 0 always holds
+        lo += l * step;
+        len = min(step, len - l * step);
+    }
+    return lo;
+}
+
+// Run table of a gathered tile (one entry per thread): entry i is tile tb + i
+// of the partitioned segment; its run, clipped to the tile's key range
+// [lo, hi), covers [rpos[i], rpos[i + 1]) of the tile (relative to lo) and
+// starts at key rsrc[i] of the partitioned segment's tiles in d.
+struct RunTab {
+    uint32_t rpos[kScatterWG + 1];
+    uint32_t rsrc[kScatterWG];
+    uint32_t wle[kScatterWG / 64];  // per wave: table tiles with P(t) <= hi
+    uint32_t more;                  // the table ends before hi: another one follows
+};
+// The gather of one tile: keys [lo, hi) of gathered segment `seg` (whose
+// runs g describes), read from the runs of tiles tb on (P(tb) <= lo).  Every
+// thread computes the same job from uniform values; thread 0 keeps the next
+// one in LDS for the next iteration.
+struct GatherJob {
+    SegGather g;
+    uint32_t n;        // keys of the segment
+    uint32_t lo, hi;
+    uint32_t tb;
+    uint32_t valid;
+    uint32_t tile;     // the tile in the level's tile space
+    uint32_t seg;      // its segment
+    uint32_t seg_end;  // first tile of the next segment
+};
+__device__ __forceinline__ SegGather uni(const SegGather& g) { return SegGather{uni(g.src), uni(g.row), uni(g.ntiles)}; }
+__device__ __forceinline__ GatherJob uni_job(const GatherJob& s) {
+    GatherJob J;
+    J.valid = uni(s.valid);
+    J.g = uni(s.g);
+    J.n = uni(s.n);
+    J.lo = uni(s.lo);
+    J.hi = uni(s.hi);
+    J.tb = uni(s.tb);
+    J.tile = uni(s.tile);
+    J.seg = uni(s.seg);
+    J.seg_end = uni(s.seg_end);
+    return J;
+}
+
+// Thread tid's row entries for tile t of g: S(b, t) and the scanned run length.
+__device__ __forceinline__ void run_rows(const LevelArgs& a, const SegGather& g, uint32_t t, uint32_t& s0,
+                                         uint32_t& pre) {
+    s0 = 0;
+    pre = 0;
+    if (t < g.ntiles) {
+        s0 = a.src_mat[g.row + t];
+        pre = a.src_pre[g.row + t];
+    }
+}
+
+// The run table of job J from tile tb on, one entry per thread (the first
+// ntab threads) from the rows it loaded for tile tb + tid (run_rows).
+// Complete after a barrier.  No LDS atomics: copies may be in flight.
+template <class K>
+__device__ __forceinline__ void run_table(RunTab& T, const LevelArgs& a, const GatherJob& J, uint32_t tb,
+                                          uint32_t s0, uint32_t pre, uint32_t tid, uint32_t ntab) {
+    constexpr uint32_t kTile = TileShape<K>::kTileKeys;
+    const uint32_t base = uni(a.src_pre[J.g.row]);
+    const uint32_t t = tb + tid;
+    const bool in = t < J.g.ntiles && tid < ntab;
+    const uint32_t P = in ? pre - base : J.n;
+    const uint32_t c = min(max(P, J.lo), J.hi);
+    if (tid < ntab) {
+        T.rpos[tid] = c - J.lo;
+        T.rsrc[tid] = t * kTile + s0 + (c - P);  // used only where the clipped run is not empty
+    }
+    const uint32_t le = (uint32_t)__popcll(__ballot(in && P <= J.hi));
+    if (lane_id() == 0) T.wle[tid >> 6] = le;
+    if (tid == 0) {
+        const uint32_t te = tb + ntab;
+        const uint32_t Pe = te < J.g.ntiles ? a.src_pre[J.g.row + te] - base : J.n;
+        T.rpos[ntab] = min(max(Pe, J.lo), J.hi) - J.lo;
+        T.more = Pe < J.hi ? 1u : 0u;
+    }
+}
+// Tiles of table T with P(t) <= hi (after the barrier that completes T).
+__device__ __forceinline__ uint32_t run_table_le(const RunTab& T) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kScatterWG / 64; ++w) s += T.wle[w];
+    return uni(s);
+}
+
+// The job of the tile after J's: the next tile of J's segment, whose runs
+// start at the last table tile with P(t) <= J.hi (nle of them counted from
+// J.tb), or the first tile of the next segment.
+template <class K>
+__device__ __forceinline__ GatherJob next_job(const LevelArgs& a, const GatherJob& J, uint32_t nle, uint32_t t_end,
+                                              uint32_t nseg, uint32_t total) {
+    constexpr uint32_t kTile = TileShape<K>::kTileKeys;
+    GatherJob N = J;
+    N.valid = 0;
+    N.g.ntiles = 0;  // (run_rows of an invalid job loads nothing)
+    if (!J.valid || J.tile + 1 >= t_end) return N;
+    N.g.ntiles = J.g.ntiles;
+    N.valid = 1;
+    N.tile = J.tile + 1;
+    if (N.tile < J.seg_end) {
+        N.lo = J.lo + kTile;
+        N.tb = J.tb + nle - 1u;
+    } else {
+        N.seg = J.seg + 1;
+        N.g = uni(a.gather[N.seg]);
+        N.n = uni(a.segs[N.seg].n);
+        N.seg_end = N.seg + 1 < nseg ? uni(a.plans[N.seg + 1].tile_begin) : total;
+        N.lo = 0;
+        N.tb = 0;
+    }
+    N.hi = min(N.lo + kTile, N.n);
+    return N;
+}
+
+// Asynchronous copies (global_load_lds) of the keys of table T that fall in
+// this wave's share of the tile, positions [w * span, (w + 1) * span), into
+// dst at their tile positions: one copy instruction per 64 dwords of the
+// share, whatever the runs.  Each lane follows its own position through the
+// runs (a pointer into T that only moves forward), so a window that spans
+// several runs costs no per-run work in the wave.  Every wave takes the same
+// share whatever the run lengths.  Complete after vm_wait_all() in every wave
+// and a barrier.
+template <class K>
+__device__ __forceinline__ void run_copy(K* dst, const RunTab& T, const LevelArgs& a, const GatherJob& J,
+                                         uint32_t tid, uint32_t ntab) {
+    constexpr uint32_t NW = kScatterWG / 64, DW = sizeof(K) / 4;
+    constexpr uint32_t kSpan = (uint32_t)TileShape<K>::kTileKeys / NW;
+    constexpr uint32_t kWin = 64 / DW;  // keys per copy instruction
+    const uint32_t lane = tid & 63u, w = uni(tid >> 6);
+    const uint32_t cnt = J.hi - J.lo;
+    const uint32_t xs = max(w * kSpan, uni(T.rpos[0]));
+    const uint32_t xe = min(min((w + 1) * kSpan, cnt), uni(T.rpos[ntab]));
+    if (xs >= xe) return;
+    // the entry holding position xs: the last i < ntab with rpos[i] <= xs
+    // (rpos[0] <= xs): 64 probes 16 apart, then 16 single ones
+    const uint32_t p1 = lane * 16u;
+    const uint32_t l1 = 63u - (uint32_t)__clzll(__ballot(p1 < ntab && T.rpos[p1] <= xs));
+    const uint32_t p2 = l1 * 16u + (lane & 15u);
+    const uint32_t l2 = 63u - (uint32_t)__clzll(__ballot(lane < 16u && p2 < ntab && T.rpos[p2] <= xs));
+    // this lane's run: [rp, rp1) of the tile, first key rs
+    uint32_t i = l1 * 16u + l2;
+    uint32_t rp = T.rpos[i], rp1 = T.rpos[i + 1], rs = T.rsrc[i];
+    const uint32_t* s0 = reinterpret_cast<const uint32_t*>((const K*)a.d + J.g.src);
+    uint32_t* d0 = reinterpret_cast<uint32_t*>(dst);
+    const uint32_t sub = lane % DW;
+#pragma nounroll
+    for (uint32_t x = xs; x < xe; x += kWin) {
+        const uint32_t pos = x + lane / DW;
+        // forward to the run holding pos; pos < xe <= rpos[ntab] stops it
+        // before the last entry
+        while (pos < xe && pos >= rp1) {
+            ++i;
+            rp = rp1;
+            rp1 = T.rpos[i + 1];
+            rs = T.rsrc[i];
+        }
+        if (pos < xe) {
+            const uint32_t* src = s0 + (size_t)(rs + (pos - rp)) * DW + sub;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(d0 + (size_t)x * DW), 4, 0, 0);
+        }
+    }
+}
+
+
 // ------------------------------------------------------------ k_tilepart
-// The tile-local partition level (the last level before the local sorts):
-// persistent over tiles like k_scatter, each work-group a contiguous run of
-// tiles.  Per tile: classify the keys, rank them within their bucket by
-// returning LDS adds, scan the counts into the in-tile bucket starts S(b, t),
-// stage the tile grouped by bucket and store it over the same positions of dn
-// (in place when the segment already lives in dn: the whole tile is in
-// registers before any of it is stored).  S(., t) and, as row nb, the tile's
-// size go to the count matrix for k_plan_local and for the leaf pass, which
-// gathers every bucket's runs from the tiles.  So this level reads and writes
-// each key once, with no count pass, and its stores are whole tiles.  Equality
-// buckets are stored too (the leaf pass fills their value into d).
-template <int DT>
+// A tile-local level: each 16 Ki-key tile is partitioned on its own, its
+// keys written back grouped by bucket to the tile's own range (no count pass,
+// no global scan), its run starts stored as one row of the count matrix and
+// its bucket totals added to bsum.  Persistent over tiles like k_scatter, each
+// work-group a contiguous run of tiles.  Per tile: classify the keys, rank
+// them within their bucket by returning LDS adds, scan the counts into the
+// in-tile bucket starts S(b, t), stage the tile grouped by bucket and store it
+// over the tile's range (in place where the segment already lives there: the
+// whole tile is in registers before any of it is stored).  S(., t) and, as
+// row nb, the tile's size go to the count matrix for the level's plan and for
+// the readers of the runs.  Equality buckets are stored too (their value is
+// filled into d once nothing reads those positions any more).
+//   tile_local 1: contiguous segments, tiles into the same range of dn;
+//   tile_local 2: contiguous segments of d, tiles in place in d, run lengths
+//                 also stored (runlen, laid out as the count matrix) so that
+//                 every bucket can be read as one gathered segment;
+//   kGather:      gathered segments (SEG_GATHER, the buckets of a tile_local 2
+//                 level), tiles into dn from the segment's start.  A tile's
+//                 keys travel to gbuf by asynchronous copies issued at the
+//                 top of the previous tile's iteration (from a run table
+//                 built out of rows loaded one iteration earlier still), so
+//                 they are in flight while that tile is partitioned; the
+//                 barriers and LDS atomics after the issue do not wait for
+//                 them.
+template <int DT, bool kGather>
 __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
     using TS = TileShape<K>;
@@ -1098,21 +1365,72 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     constexpr uint32_t kTile = (uint32_t)kScatterWG * KPT;
     static_assert(TS::kTileKeys == (int)kTile, "one tile per pass of the work-group");
     static_assert(kTile <= 65536 && kSplStride <= 256, "bucket and rank share one word");
-    constexpr bool kEarlyPrefetch = DT == DT_U32;  // keys need no transform
+    constexpr bool kEarlyPrefetch = DT == DT_U32 && !kGather;  // keys need no transform
     __shared__ TileLds<K> L;
     __shared__ K stage[kTile];
     __shared__ uint32_t wsum[kScatterWG / 64];
     __shared__ SegRec s_seg;
     __shared__ SegPlan s_plan;
+    // gathered input: the next tile's keys (an LDS variable of its own, so that
+    // plain LDS accesses of the others need not wait for the copies), its run
+    // table, and the jobs of the next two tiles (slot tile & 1)
+    __shared__ K gbuf[kGather ? kTile : 1];
+    __shared__ std::conditional_t<kGather, RunTab, uint32_t> T;
+    __shared__ GatherJob s_job[kGather ? 2 : 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
     const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
+    // run-table entries (one per thread; fewer only to drive the multi-table path)
+    const uint32_t ntab = kGather ? min(max(a.run_tab, 64u), (uint32_t)kScatterWG) : 0u;
     uint32_t seg_i = 0, seg_next = 0;
     if (tid == 0 && t0 < t1) {
         seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
         seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+    }
+    uint32_t rr_s0 = 0, rr_pre = 0;  // gathered input: rows of the table of the tile after next
+    if constexpr (kGather) {
+        // the first tile's keys, gathered before the loop
+        if (t0 < t1) {
+            if (tid == 0) {
+                GatherJob& J = s_job[t0 & 1];
+                J.g = a.gather[seg_i];
+                J.n = a.segs[seg_i].n;
+                J.lo = (t0 - a.plans[seg_i].tile_begin) * kTile;
+                J.hi = min(J.lo + kTile, J.n);
+                J.tb = 0;
+                J.valid = 1;
+                J.tile = t0;
+                J.seg = seg_i;
+                J.seg_end = seg_next;
+            }
+            __syncthreads();
+            if (uni(s_job[t0 & 1].lo) > 0) {
+                if (tid < 64) {
+                    const uint32_t tb = run_find(a, uni(s_job[t0 & 1].g), uni(s_job[t0 & 1].lo));
+                    if (tid == 0) s_job[t0 & 1].tb = tb;
+                }
+                __syncthreads();
+            }
+            const GatherJob J = uni_job(s_job[t0 & 1]);
+            uint32_t tb = J.tb, nle = 0;
+            for (;;) {
+                uint32_t r0, rp;
+                run_rows(a, J.g, tb + tid, r0, rp);
+                run_table<K>(T, a, J, tb, r0, rp, tid, ntab);
+                __syncthreads();
+                nle += run_table_le(T);
+                run_copy<K>(gbuf, T, a, J, tid, ntab);
+                const bool more = uni(T.more) != 0;
+                __syncthreads();  // every wave is done with T
+                if (!more) break;
+                tb += ntab;
+            }
+            const GatherJob J1 = next_job<K>(a, J, nle, t1, nseg, total);
+            if (tid == 0) s_job[(t0 + 1) & 1] = J1;
+            run_rows(a, J1.g, J1.tb + tid, rr_s0, rr_pre);
+        }
     }
     uint32_t cur = 0xFFFFFFFFu;
     K keys[KPT];
@@ -1129,7 +1447,8 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             }
             L.seg = seg_i;
         }
-        __syncthreads();  // the previous tile's stores have read stage
+        if constexpr (kGather) vm_wait_all();  // this wave's copies of this tile's keys
+        __syncthreads();  // the previous tile's stores have read stage; gbuf complete
         const uint32_t s = uni(L.seg);
         if (s != cur) {
             load_splitters<K>(a, s, L);
@@ -1145,13 +1464,23 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         const uint32_t tl = t - p.tile_begin;
         if (ti < (uint32_t)kSplStride) L.hist[ti] = 0;
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
-        K* dst = (K*)a.dn + seg.start;
+        // tile_local 2 partitions in place; the others write dn
+        K* dst = (a.tile_local == 2 ? (K*)src : (K*)a.dn) + seg.start;
         const bool raw = seg.flags & SEG_RAW;
         const uint32_t lo = tl * kTile;
         const uint32_t hi = min(lo + kTile, seg.n);
         const uint32_t cnt = hi - lo;
         const K* base = src + seg.start;
-        if (!have_keys) {
+        if constexpr (kGather) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) {
+                const K v = gbuf[j * kScatterWG + ti];
+                keys[j] = (j * kScatterWG + ti < cnt) ? v : K(0);
+            }
+            // the next tile's run table, from the rows loaded last iteration
+            const GatherJob J1 = uni_job(s_job[(t + 1) & 1]);
+            if (J1.valid) run_table<K>(T, a, J1, J1.tb, rr_s0, rr_pre, ti, ntab);
+        } else if (!have_keys) {
             K raw_keys[KPT];
 #pragma unroll
             for (int j = 0; j < KPT; ++j) raw_keys[j] = base[min(lo + j * kScatterWG + ti, hi - 1u)];
@@ -1161,7 +1490,41 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 keys[j] = (lo + j * kScatterWG + ti < hi) ? v : K(0);
             }
         }
-        __syncthreads();  // counters cleared
+        __syncthreads();  // counters cleared (gathered input: gbuf read, run table complete)
+        if constexpr (kGather) {
+            // the job and rows of the tile after next, then the next tile's
+            // copies, in flight until the top of the next iteration.  The row
+            // loads go first and unconditionally: a load issued after the
+            // copies, or a register assigned on one branch only, would make
+            // the compiler wait for the copies here.  When this tile's runs
+            // need further tables, the tile after next starts from the first
+            // table's last tile (P <= hi still holds there).
+            const GatherJob J1 = uni_job(s_job[(t + 1) & 1]);
+            GatherJob J2 = J1;
+            J2.valid = 0;
+            J2.g.ntiles = 0;
+            if (J1.valid) J2 = next_job<K>(a, J1, run_table_le(T), t1, nseg, total);
+            if (ti == 0) s_job[t & 1] = J2;
+            run_rows(a, J2.g, J2.tb + ti, rr_s0, rr_pre);
+            if (J1.valid) {
+                run_copy<K>(gbuf, T, a, J1, ti, ntab);
+                if (uni(T.more)) {
+                    // runs shorter than a table covers (rare): further tables, in turn
+                    uint32_t tb = J1.tb;
+                    for (;;) {
+                        const bool more = uni(T.more) != 0;
+                        __syncthreads();  // every wave is done with T
+                        if (!more) break;
+                        tb += ntab;
+                        uint32_t r0, rp;
+                        run_rows(a, J1.g, tb + ti, r0, rp);
+                        run_table<K>(T, a, J1, tb, r0, rp, ti, ntab);
+                        __syncthreads();
+                        run_copy<K>(gbuf, T, a, J1, ti, ntab);
+                    }
+                }
+            }
+        }
         // bucket (< 256) and rank within the bucket (< kTile), packed
         uint32_t br[KPT];
         {
@@ -1170,28 +1533,31 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
             for (int j = 0; j < KPT; ++j) valid[j] = lo + j * kScatterWG + ti < hi;
-            hist_rank_n<KPT>(L.hist, bk, valid, rk);
+            hist_rank_n<KPT, kGather>(L.hist, bk, valid, rk);
 #pragma unroll
             for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
         }
-        __syncthreads();  // every count is in hist
+        if constexpr (kGather) lds_sync();  // every count is in hist (the copies stay in flight)
+        else __syncthreads();             // every count is in hist
         // in-tile bucket starts: thread b owns bucket b
         {
             const uint32_t mycnt = (ti < p.nbuckets) ? L.hist[ti] : 0u;
             uint32_t tot;
-            const uint32_t st = block_excl_scan<kScatterWG, false, false>(mycnt, wsum, &tot);
+            const uint32_t st = block_excl_scan<kScatterWG, kGather, false>(mycnt, wsum, &tot);
             uint32_t* col = a.mat + p.mat_base + tl;
             if (ti < p.nbuckets) {
                 L.hist[ti] = st;
                 col[(size_t)ti * p.ntiles] = st;
+                if (a.tile_local == 2) a.runlen[p.mat_base + tl + (size_t)ti * p.ntiles] = mycnt;
                 if (mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
             } else if (ti == p.nbuckets) {
                 col[(size_t)ti * p.ntiles] = cnt;  // row nb: the tile's size
+                if (a.tile_local == 2) a.runlen[p.mat_base + tl + (size_t)ti * p.ntiles] = 0;
             }
         }
         // prefetch: the next tile when it is this work-group's and in the same
         // segment (uint32 keys now, float keys after the staging, as k_scatter)
-        const bool next_tile = t + 1 < t1 && tl + 1 < p.ntiles;
+        const bool next_tile = !kGather && t + 1 < t1 && tl + 1 < p.ntiles;
         const uint32_t nlo = lo + kTile;
         const uint32_t nhi = next_tile ? min(nlo + kTile, seg.n) : 0u;
         K nxt[KPT];
@@ -1202,11 +1568,12 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 nxt[j] = (idx < nhi) ? base[idx] : K(0);
             }
         }
-        __syncthreads();  // bucket starts in hist
+        if constexpr (kGather) lds_sync();  // bucket starts in hist
+        else __syncthreads();             // bucket starts in hist
 #pragma unroll
         for (int j = 0; j < KPT; ++j)
             if (lo + j * kScatterWG + ti < hi) stage[L.hist[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
-        if constexpr (!kEarlyPrefetch) {
+        if constexpr (!kEarlyPrefetch && !kGather) {
             if (next_tile) {  // uniform
 #pragma unroll
                 for (int j = 0; j < KPT; ++j) nxt[j] = base[min(nlo + j * kScatterWG + ti, nhi - 1u)];
@@ -1215,24 +1582,27 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
             }
         }
-        __syncthreads();  // the tile is staged
+        if constexpr (kGather) lds_sync();  // the tile is staged
+        else __syncthreads();             // the tile is staged
         // the tile back, grouped by bucket: coalesced stores of whole tiles
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
             const uint32_t i = j * kScatterWG + ti;
             if (i < cnt) dst[lo + i] = stage[i];
         }
-        if constexpr (kEarlyPrefetch) {
+        if constexpr (!kGather) {
+            if constexpr (kEarlyPrefetch) {
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
-        } else {
+                for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+            } else {
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const K v = raw ? to_key<DT>(nxt[j]) : nxt[j];
-                keys[j] = (nlo + j * kScatterWG + ti < nhi) ? v : K(0);
+                for (int j = 0; j < KPT; ++j) {
+                    const K v = raw ? to_key<DT>(nxt[j]) : nxt[j];
+                    keys[j] = (nlo + j * kScatterWG + ti < nhi) ? v : K(0);
+                }
             }
+            have_keys = next_tile;
         }
-        have_keys = next_tile;
     }
 }
 
@@ -1678,7 +2048,7 @@ __global__ __launch_bounds__(kLeafWG) void k_leaf_small(LeafArgs a) {
         const uint32_t nt = min(a.lv.ctl->nfill, a.lv.fill_cap);
         for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
             const FillTask f = a.lv.fill[t];
-            const K v = from_key<DT>(((const K*)a.lv.spl)[f.spl_index]);
+            const K v = from_key<DT>((K)f.key);
             for (uint32_t i = tid; i < f.n; i += kLeafWG) dout[f.start + i] = v;
         }
     }

@@ -107,6 +107,7 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_uint32), _P, _P, _P,
                                            ctypes.POINTER(ctypes.c_uint32)]),
     "gqs_set_queue_limit": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "gqs_set_level_scheme": (ctypes.c_int, [_P, ctypes.c_int]),
     "gqs_graph_status": (ctypes.c_int, [_P]),
     "gqs_device_report": (ctypes.c_int, [_P, ctypes.POINTER(DeviceProps)]),
     "gqs_server_resolve_config": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(ServerConfig)]),
@@ -361,6 +362,12 @@ class Context:
     def set_queue_limit(self, segments):
         """Testing hook: cap the segment queue of later sorts (0: no cap)."""
         _check(self._lib.gqs_set_queue_limit(self._h, segments), self)
+
+    def set_level_scheme(self, scheme):
+        """Testing hook: 0 runs two tile-local levels where a sort plans two
+        levels and a spare (the default), 1 a contiguous level before the
+        tile-local one, 2 as 0 with 64-entry run tables (the multi-table path)."""
+        _check(self._lib.gqs_set_level_scheme(self._h, scheme), self)
 
     def device_report(self):
         """hipGetDeviceProperties report (the reference's show_CL output)."""

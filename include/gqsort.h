@@ -217,6 +217,16 @@ gqs_status gqs_get_stats(gqs_ctx* ctx, gqs_stats* out);
  * cap), so that the overflow path can be driven from the host. */
 gqs_status gqs_set_queue_limit(gqs_ctx* ctx, uint32_t segments);
 
+/* Testing hook, no counterpart in the reference: the level schedule of later
+ * sorts of one range that plan two partition levels and a spare.  0 (the
+ * default): both levels tile-local, the first in place, the second reading
+ * the first one's runs (no count pass).  1: a contiguous level (count, scan,
+ * scatter) before the tile-local one.  2: as 0, with the second level reading
+ * its runs through 64-entry run tables instead of one entry per thread, so
+ * that tests reach the path taken by runs too short for one table.  All give
+ * the same output; the hook lets tests and benchmarks compare them. */
+gqs_status gqs_set_level_scheme(gqs_ctx* ctx, int scheme);
+
 /* Per-kernel timing with HIP events on the sort's own stream.  While enabled,
  * every launch is bracketed by events; gqs_kernel_times() synchronises and
  * returns, per kernel kind, the number of launches and total milliseconds.
