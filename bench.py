@@ -68,13 +68,20 @@ def kernel_roofline(ctx, dt, d, dn, pristine, nbytes, n, stream, kb, workload):
     ctx.set_profiling(True)
     ctx.sort_device(dt, d, dn, n, stream)
     times = ctx.kernel_times()
+    trace = ctx.kernel_trace()
     ctx.set_profiling(False)
     st = ctx.stats()
     level_keys = [int(k) for k in st.level_keys]
-    # the level before the spare one partitions tile-locally (k_tilepart),
-    # the others count and scatter
-    local = st.levels_launched - 2 if st.levels_launched >= 2 else None
-    local_keys = level_keys[local] if local is not None and local < len(level_keys) else 0
+    # levels that partition tile-locally (k_tilepart) and those that count and
+    # scatter, told apart by the launches of each level (one splitter launch
+    # opens every level)
+    lvl, local_levels = -1, set()
+    for kind, _ in trace:
+        if kind == "splitters":
+            lvl += 1
+        elif kind == "tilepart":
+            local_levels.add(lvl)
+    local_keys = sum(level_keys[i] for i in local_levels if 0 <= i < len(level_keys))
     keys_partitioned = sum(level_keys) - local_keys
     # keys handled per kind over the whole sort
     keys = {"count": keys_partitioned, "scatter": keys_partitioned, "leaf": n, "tilepart": local_keys}

@@ -208,60 +208,17 @@ __device__ __forceinline__ void hist_add_asm(uint32_t* hist, uint32_t b) {
         lds_add(&hist[b], 1u);
     }
 }
-// Eight returning LDS adds of per-lane values in one asm statement, waited
-// for together (as lds_add_rtn4).
-__device__ __forceinline__ void lds_add_rtn8v(const uint32_t (&ad)[8], const uint32_t (&v)[8], uint32_t (&r)[8]) {
-    asm volatile(
-        "ds_add_rtn_u32 %0, %8, %16\n\t"
-        "ds_add_rtn_u32 %1, %9, %17\n\t"
-        "ds_add_rtn_u32 %2, %10, %18\n\t"
-        "ds_add_rtn_u32 %3, %11, %19\n\t"
-        "ds_add_rtn_u32 %4, %12, %20\n\t"
-        "ds_add_rtn_u32 %5, %13, %21\n\t"
-        "ds_add_rtn_u32 %6, %14, %22\n\t"
-        "ds_add_rtn_u32 %7, %15, %23\n\t"
-        "s_waitcnt lgkmcnt(0)"
-        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]), "=&v"(r[7])
-        : "v"(ad[0]), "v"(ad[1]), "v"(ad[2]), "v"(ad[3]), "v"(ad[4]), "v"(ad[5]), "v"(ad[6]), "v"(ad[7]),
-          "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(v[4]), "v"(v[5]), "v"(v[6]), "v"(v[7])
-        : "memory");
-}
-
 // hist_rank for N keys per thread at once: every key's atomic is issued
 // before any result is waited for.  Invalid keys (valid[j] false) take no
 // slot and get rank 0.
-// With kAsm the adds are inline assembly (lds_add_rtn*), for kernels that
-// have global->LDS copies in flight.
-template <int N, bool kAsm = false>
+template <int N>
 __device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[N], const bool (&valid)[N],
                                             uint32_t (&r)[N]) {
-    // one add; the returning asm form waits for its own result
-    auto add = [&](uint32_t* p, uint32_t v) -> uint32_t {
-        if constexpr (kAsm) return lds_add_rtn(p, v);
-        else return atomicAdd(p, v);
-    };
     if (!clustered(b[0], valid[0])) {
-        if constexpr (kAsm) {
-            static_assert(N % 8 == 0, "adds go in eights");
-            // keys of invalid slots add 0 (to bucket 0): no effect
 #pragma unroll
-            for (int j = 0; j < N; j += 8) {
-                uint32_t ad[8], v[8], rr[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    ad[u] = lds_addr(&hist[valid[j + u] ? b[j + u] : 0u]);
-                    v[u] = valid[j + u] ? 1u : 0u;
-                }
-                lds_add_rtn8v(ad, v, rr);
-#pragma unroll
-                for (int u = 0; u < 8; ++u) r[j + u] = valid[j + u] ? rr[u] : 0u;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-                r[j] = 0;
-                if (valid[j]) r[j] = atomicAdd(&hist[b[j]], 1u);
-            }
+        for (int j = 0; j < N; ++j) {
+            r[j] = 0;
+            if (valid[j]) r[j] = atomicAdd(&hist[b[j]], 1u);
         }
         return;
     }
@@ -279,7 +236,7 @@ __device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[
                 total += (uint32_t)__popcll(vm[j]);
             }
             uint32_t base = 0;
-            if (lane_id() == l0) base = add(&hist[b0], total);
+            if (lane_id() == l0) base = atomicAdd(&hist[b0], total);
             base = (uint32_t)__builtin_amdgcn_readlane((int)base, l0);
 #pragma unroll
             for (int j = 0; j < N; ++j) r[j] = valid[j] ? base + before[j] + mbcnt(vm[j]) : 0u;
@@ -296,10 +253,10 @@ __device__ __forceinline__ void hist_rank_n(uint32_t* hist, const uint32_t (&b)[
         const Peel pl = peel2(b[j], valid[j]);
         r[j] = 0;
         if (pl.two) {
-            if (lane == pl.l0) r[j] = add(&hist[b[j]], (uint32_t)__popcll(pl.m0));
-            if (pl.m1 && lane == pl.l1) r[j] = add(&hist[b[j]], (uint32_t)__popcll(pl.m1));
+            if (lane == pl.l0) r[j] = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m0));
+            if (pl.m1 && lane == pl.l1) r[j] = atomicAdd(&hist[b[j]], (uint32_t)__popcll(pl.m1));
         } else if (valid[j]) {
-            r[j] = add(&hist[b[j]], 1u);
+            r[j] = atomicAdd(&hist[b[j]], 1u);
         }
     }
 #pragma unroll
@@ -531,9 +488,11 @@ __device__ void global_bitonic(K* buf, uint64_t n, uint32_t tid, uint32_t nthrea
 // of the segment.  The run of tile t starts at P(t) = pre[t] - pre[0] in the
 // segment's key order; the key is in the last tile t with P(t) <= pos (that
 // run is not empty: P(t + 1) > pos, or t is the last tile).
+// The run starts P come from pre: the scanned run lengths in global memory,
+// or a copy of that row staged in LDS.
 template <class K>
-__device__ __forceinline__ uint64_t gathered_index(const LevelArgs& a, const SegGather& g, uint32_t pos) {
-    const uint32_t* pre = a.src_pre + g.row;
+__device__ __forceinline__ uint64_t gathered_index(const LevelArgs& a, const SegGather& g, uint32_t pos,
+                                                   const uint32_t* pre) {
     const uint32_t base = pre[0];
     uint32_t lo = 0, hi = g.ntiles;  // the tile is in [lo, hi)
     while (hi - lo > 1) {
@@ -551,6 +510,10 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
     __shared__ K cand[kSplStride];
     __shared__ uint32_t wsum[kSplWG / 64];
     __shared__ uint16_t lutw[kLutCells + 1];
+    // gathered segments: the row of scanned run lengths, staged so that each
+    // sample's search reads LDS (rows of at most kPreRow tiles)
+    constexpr uint32_t kPreRow = 16384;
+    __shared__ uint32_t prow[kPreRow];
     const uint32_t tid = threadIdx.x;
     const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);  // clamped: see k_tileplan
     for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
@@ -561,8 +524,14 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
         // key at position pos of the segment (a gathered segment: through its runs)
         const bool gathered = seg.flags & SEG_GATHER;
         const SegGather gs = gathered ? a.gather[s] : SegGather{0, 0, 0};
+        const bool staged = gathered && gs.ntiles <= kPreRow;
+        if (staged) {
+            for (uint32_t i = tid; i < gs.ntiles; i += kSplWG) prow[i] = a.src_pre[gs.row + i];
+            __syncthreads();
+        }
+        const uint32_t* pre = staged ? prow : a.src_pre + gs.row;
         auto key_at = [&](uint32_t pos) -> K {
-            if (gathered) return src[gathered_index<K>(a, gs, pos)];
+            if (gathered) return src[gathered_index<K>(a, gs, pos, pre)];
             return load_key<DT>(src, seg.start + pos, raw);
         };
         K* spl = (K*)a.spl + (size_t)s * kSplStride;
@@ -1103,7 +1072,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_scatter(LevelArgs a) {
                 for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
             }
         }
-        __syncthreads();
+        lds_sync();  // staged (LDS only: the prefetched keys stay in flight)
         // coalesced stores of the staged runs, W slots per thread at a time:
         // bucket ids, then offsets and keys, each batch of LDS reads issued
         // together (slots past cnt are read but not stored)
@@ -1281,20 +1250,18 @@ __device__ __forceinline__ GatherJob next_job(const LevelArgs& a, const GatherJo
     return N;
 }
 
-// Asynchronous copies (global_load_lds) of the keys of table T that fall in
-// this wave's share of the tile, positions [w * span, (w + 1) * span), into
-// dst at their tile positions: one copy instruction per 64 dwords of the
-// share, whatever the runs.  Each lane follows its own position through the
-// runs (a pointer into T that only moves forward), so a window that spans
-// several runs costs no per-run work in the wave.  Every wave takes the same
-// share whatever the run lengths.  Complete after vm_wait_all() in every wave
-// and a barrier.
-template <class K>
-__device__ __forceinline__ void run_copy(K* dst, const RunTab& T, const LevelArgs& a, const GatherJob& J,
+// The keys of table T that fall in this wave's share of the tile, positions
+// [w * span, (w + 1) * span) with span = KPT * 64, loaded into the registers
+// of their slots: slot j of lane l holds position w * span + j * 64 + l, so
+// every load instruction reads 64 consecutive positions (one run, or the
+// end of one and the start of the next: coalesced either way).  Each lane
+// follows its own positions through the runs with a pointer into T that
+// only moves forward.  Slots outside the table's positions are left as they
+// are (another table covers them).
+template <class K, int KPT>
+__device__ __forceinline__ void run_load(K (&v)[KPT], const RunTab& T, const LevelArgs& a, const GatherJob& J,
                                          uint32_t tid, uint32_t ntab) {
-    constexpr uint32_t NW = kScatterWG / 64, DW = sizeof(K) / 4;
-    constexpr uint32_t kSpan = (uint32_t)TileShape<K>::kTileKeys / NW;
-    constexpr uint32_t kWin = 64 / DW;  // keys per copy instruction
+    constexpr uint32_t kSpan = (uint32_t)KPT * 64u;
     const uint32_t lane = tid & 63u, w = uni(tid >> 6);
     const uint32_t cnt = J.hi - J.lo;
     const uint32_t xs = max(w * kSpan, uni(T.rpos[0]));
@@ -1306,31 +1273,25 @@ __device__ __forceinline__ void run_copy(K* dst, const RunTab& T, const LevelArg
     const uint32_t l1 = 63u - (uint32_t)__clzll(__ballot(p1 < ntab && T.rpos[p1] <= xs));
     const uint32_t p2 = l1 * 16u + (lane & 15u);
     const uint32_t l2 = 63u - (uint32_t)__clzll(__ballot(lane < 16u && p2 < ntab && T.rpos[p2] <= xs));
-    // this lane's run: [rp, rp1) of the tile, first key rs
     uint32_t i = l1 * 16u + l2;
     uint32_t rp = T.rpos[i], rp1 = T.rpos[i + 1], rs = T.rsrc[i];
-    const uint32_t* s0 = reinterpret_cast<const uint32_t*>((const K*)a.d + J.g.src);
-    uint32_t* d0 = reinterpret_cast<uint32_t*>(dst);
-    const uint32_t sub = lane % DW;
-#pragma nounroll
-    for (uint32_t x = xs; x < xe; x += kWin) {
-        const uint32_t pos = x + lane / DW;
-        // forward to the run holding pos; pos < xe <= rpos[ntab] stops it
-        // before the last entry
-        while (pos < xe && pos >= rp1) {
-            ++i;
-            rp = rp1;
-            rp1 = T.rpos[i + 1];
-            rs = T.rsrc[i];
-        }
-        if (pos < xe) {
-            const uint32_t* src = s0 + (size_t)(rs + (pos - rp)) * DW + sub;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)(d0 + (size_t)x * DW), 4, 0, 0);
+    const K* s0 = (const K*)a.d + J.g.src;
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const uint32_t pos = w * kSpan + (uint32_t)j * 64u + lane;
+        if (pos >= xs && pos < xe) {
+            // forward to the run holding pos; pos < xe <= rpos[ntab] stops it
+            // before the last entry
+            while (pos >= rp1) {
+                ++i;
+                rp = rp1;
+                rp1 = T.rpos[i + 1];
+                rs = T.rsrc[i];
+            }
+            v[j] = s0[rs + (pos - rp)];
         }
     }
 }
-
 
 // ------------------------------------------------------------ k_tilepart
 // A tile-local level: each 16 Ki-key tile is partitioned on its own, its
@@ -1351,12 +1312,13 @@ __device__ __forceinline__ void run_copy(K* dst, const RunTab& T, const LevelArg
 //                 every bucket can be read as one gathered segment;
 //   kGather:      gathered segments (SEG_GATHER, the buckets of a tile_local 2
 //                 level), tiles into dn from the segment's start.  A tile's
-//                 keys travel to gbuf by asynchronous copies issued at the
-//                 top of the previous tile's iteration (from a run table
-//                 built out of rows loaded one iteration earlier still), so
-//                 they are in flight while that tile is partitioned; the
-//                 barriers and LDS atomics after the issue do not wait for
-//                 them.
+//                 keys are loaded into registers while the previous tile is
+//                 partitioned (as the contiguous kinds prefetch theirs),
+//                 from a run table built at the top of that iteration out of
+//                 rows loaded one iteration earlier still; each wave takes
+//                 a contiguous share of the tile's positions (the slot
+//                 layout differs from the contiguous kinds', which the
+//                 partition does not depend on).
 template <int DT, bool kGather>
 __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     using K = typename KeyOf<DT>::K;
@@ -1371,10 +1333,8 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     __shared__ uint32_t wsum[kScatterWG / 64];
     __shared__ SegRec s_seg;
     __shared__ SegPlan s_plan;
-    // gathered input: the next tile's keys (an LDS variable of its own, so that
-    // plain LDS accesses of the others need not wait for the copies), its run
-    // table, and the jobs of the next two tiles (slot tile & 1)
-    __shared__ K gbuf[kGather ? kTile : 1];
+    // gathered input: the next tile's run table and the jobs of the next two
+    // tiles (slot tile & 1)
     __shared__ std::conditional_t<kGather, RunTab, uint32_t> T;
     __shared__ GatherJob s_job[kGather ? 2 : 1];
     const uint32_t tid = threadIdx.x;
@@ -1390,6 +1350,9 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
     }
     uint32_t rr_s0 = 0, rr_pre = 0;  // gathered input: rows of the table of the tile after next
+    K keys[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) keys[j] = K(0);
     if constexpr (kGather) {
         // the first tile's keys, gathered before the loop
         if (t0 < t1) {
@@ -1421,7 +1384,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 run_table<K>(T, a, J, tb, r0, rp, tid, ntab);
                 __syncthreads();
                 nle += run_table_le(T);
-                run_copy<K>(gbuf, T, a, J, tid, ntab);
+                run_load<K, KPT>(keys, T, a, J, tid, ntab);
                 const bool more = uni(T.more) != 0;
                 __syncthreads();  // every wave is done with T
                 if (!more) break;
@@ -1433,7 +1396,6 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         }
     }
     uint32_t cur = 0xFFFFFFFFu;
-    K keys[KPT];
     bool have_keys = false;  // keys[] already hold this tile's keys
     for (uint32_t t = t0; t < t1; ++t) {
         // tid, opaque to the compiler (LDS addresses derived from it are
@@ -1447,8 +1409,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             }
             L.seg = seg_i;
         }
-        if constexpr (kGather) vm_wait_all();  // this wave's copies of this tile's keys
-        __syncthreads();  // the previous tile's stores have read stage; gbuf complete
+        __syncthreads();  // the previous tile's stores have read stage
         const uint32_t s = uni(L.seg);
         if (s != cur) {
             load_splitters<K>(a, s, L);
@@ -1471,13 +1432,14 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         const uint32_t hi = min(lo + kTile, seg.n);
         const uint32_t cnt = hi - lo;
         const K* base = src + seg.start;
+        // position in the tile of slot j of this thread (see run_load)
+        auto slot_pos = [&](int j) -> uint32_t {
+            if constexpr (kGather) return (ti >> 6) * (uint32_t)(KPT * 64) + (uint32_t)j * 64u + (ti & 63u);
+            else return (uint32_t)j * kScatterWG + ti;
+        };
         if constexpr (kGather) {
-#pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const K v = gbuf[j * kScatterWG + ti];
-                keys[j] = (j * kScatterWG + ti < cnt) ? v : K(0);
-            }
-            // the next tile's run table, from the rows loaded last iteration
+            // this tile's keys arrived in keys[] during the previous iteration;
+            // the next tile's run table, from the rows loaded then too
             const GatherJob J1 = uni_job(s_job[(t + 1) & 1]);
             if (J1.valid) run_table<K>(T, a, J1, J1.tb, rr_s0, rr_pre, ti, ntab);
         } else if (!have_keys) {
@@ -1490,15 +1452,13 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 keys[j] = (lo + j * kScatterWG + ti < hi) ? v : K(0);
             }
         }
-        __syncthreads();  // counters cleared (gathered input: gbuf read, run table complete)
+        __syncthreads();  // counters cleared (gathered input: run table complete)
         if constexpr (kGather) {
-            // the job and rows of the tile after next, then the next tile's
-            // copies, in flight until the top of the next iteration.  The row
-            // loads go first and unconditionally: a load issued after the
-            // copies, or a register assigned on one branch only, would make
-            // the compiler wait for the copies here.  When this tile's runs
-            // need further tables, the tile after next starts from the first
-            // table's last tile (P <= hi still holds there).
+            // the job and rows of the tile after next (loaded unconditionally:
+            // a register assigned on one branch only costs a wait here).  When
+            // the next tile's runs need further tables, the tile after next
+            // starts from the first table's last tile (P <= hi still holds
+            // there).
             const GatherJob J1 = uni_job(s_job[(t + 1) & 1]);
             GatherJob J2 = J1;
             J2.valid = 0;
@@ -1506,24 +1466,6 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             if (J1.valid) J2 = next_job<K>(a, J1, run_table_le(T), t1, nseg, total);
             if (ti == 0) s_job[t & 1] = J2;
             run_rows(a, J2.g, J2.tb + ti, rr_s0, rr_pre);
-            if (J1.valid) {
-                run_copy<K>(gbuf, T, a, J1, ti, ntab);
-                if (uni(T.more)) {
-                    // runs shorter than a table covers (rare): further tables, in turn
-                    uint32_t tb = J1.tb;
-                    for (;;) {
-                        const bool more = uni(T.more) != 0;
-                        __syncthreads();  // every wave is done with T
-                        if (!more) break;
-                        tb += ntab;
-                        uint32_t r0, rp;
-                        run_rows(a, J1.g, tb + ti, r0, rp);
-                        run_table<K>(T, a, J1, tb, r0, rp, ti, ntab);
-                        __syncthreads();
-                        run_copy<K>(gbuf, T, a, J1, ti, ntab);
-                    }
-                }
-            }
         }
         // bucket (< 256) and rank within the bucket (< kTile), packed
         uint32_t br[KPT];
@@ -1532,18 +1474,17 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             bool valid[KPT];
             classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) valid[j] = lo + j * kScatterWG + ti < hi;
-            hist_rank_n<KPT, kGather>(L.hist, bk, valid, rk);
+            for (int j = 0; j < KPT; ++j) valid[j] = slot_pos(j) < cnt;
+            hist_rank_n<KPT>(L.hist, bk, valid, rk);
 #pragma unroll
             for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
         }
-        if constexpr (kGather) lds_sync();  // every count is in hist (the copies stay in flight)
-        else __syncthreads();             // every count is in hist
+        __syncthreads();  // every count is in hist
         // in-tile bucket starts: thread b owns bucket b
         {
             const uint32_t mycnt = (ti < p.nbuckets) ? L.hist[ti] : 0u;
             uint32_t tot;
-            const uint32_t st = block_excl_scan<kScatterWG, kGather, false>(mycnt, wsum, &tot);
+            const uint32_t st = block_excl_scan<kScatterWG, false, false>(mycnt, wsum, &tot);
             uint32_t* col = a.mat + p.mat_base + tl;
             if (ti < p.nbuckets) {
                 L.hist[ti] = st;
@@ -1568,11 +1509,36 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 nxt[j] = (idx < nhi) ? base[idx] : K(0);
             }
         }
-        if constexpr (kGather) lds_sync();  // bucket starts in hist
-        else __syncthreads();             // bucket starts in hist
+        if constexpr (kGather) {
+            // the next tile's keys, through the run table built at the top
+            const GatherJob J1 = uni_job(s_job[(t + 1) & 1]);
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
+            if (J1.valid) {
+                run_load<K, KPT>(nxt, T, a, J1, ti, ntab);
+                if (uni(T.more)) {
+                    // runs shorter than a table covers (rare): further tables, in turn
+                    uint32_t tb = J1.tb;
+                    for (;;) {
+                        const bool more = uni(T.more) != 0;
+                        __syncthreads();  // every wave is done with T
+                        if (!more) break;
+                        tb += ntab;
+                        uint32_t r0, rp;
+                        run_rows(a, J1.g, tb + ti, r0, rp);
+                        run_table<K>(T, a, J1, tb, r0, rp, ti, ntab);
+                        __syncthreads();
+                        run_load<K, KPT>(nxt, T, a, J1, ti, ntab);
+                    }
+                }
+            }
+        }
+        // LDS-only barriers from here on: the next tile's keys stay in flight
+        // (the fence of __syncthreads() would wait for them)
+        lds_sync();  // bucket starts in hist
 #pragma unroll
         for (int j = 0; j < KPT; ++j)
-            if (lo + j * kScatterWG + ti < hi) stage[L.hist[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
+            if (slot_pos(j) < cnt) stage[L.hist[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
         if constexpr (!kEarlyPrefetch && !kGather) {
             if (next_tile) {  // uniform
 #pragma unroll
@@ -1582,15 +1548,17 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
             }
         }
-        if constexpr (kGather) lds_sync();  // the tile is staged
-        else __syncthreads();             // the tile is staged
+        lds_sync();  // the tile is staged
         // the tile back, grouped by bucket: coalesced stores of whole tiles
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
             const uint32_t i = j * kScatterWG + ti;
             if (i < cnt) dst[lo + i] = stage[i];
         }
-        if constexpr (!kGather) {
+        if constexpr (kGather) {
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
+        } else {
             if constexpr (kEarlyPrefetch) {
 #pragma unroll
                 for (int j = 0; j < KPT; ++j) keys[j] = nxt[j];
