@@ -529,9 +529,14 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
             for (uint32_t i = tid; i < gs.ntiles; i += kSplWG) prow[i] = a.src_pre[gs.row + i];
             __syncthreads();
         }
-        const uint32_t* pre = staged ? prow : a.src_pre + gs.row;
+        // (the staged and the global search are separate calls: one pointer
+        // that may be either would be read by flat loads, which wait for every
+        // outstanding memory operation)
         auto key_at = [&](uint32_t pos) -> K {
-            if (gathered) return src[gathered_index<K>(a, gs, pos, pre)];
+            if (gathered) {
+                if (staged) return src[gathered_index<K>(a, gs, pos, prow)];
+                return src[gathered_index<K>(a, gs, pos, a.src_pre + gs.row)];
+            }
             return load_key<DT>(src, seg.start + pos, raw);
         };
         K* spl = (K*)a.spl + (size_t)s * kSplStride;
@@ -1901,19 +1906,26 @@ __device__ __forceinline__ LeafDesc uniform_desc(const LeafDesc& d) {
 }
 
 // Rows of a gathered leaf staged in LDS ahead of its copy: S(b, t) for the
-// first kRowsLds tiles, then S(b + 1, t).
-constexpr uint32_t kRowsLds = 64;
+// first kRowsLds tiles, then S(b + 1, t).  A leaf of more tiles reads its rows
+// from global memory while it is copied, which waits for every outstanding
+// memory operation: twice the expected tile count of a segment (1 Mi keys of
+// 16 Ki-key tiles at 2^28 keys) keeps that off the common path.
+constexpr uint32_t kRowsLds = 128;
 
 // Asynchronous copy of a gathered leaf into LDS (global_load_lds, no
 // registers): run t of the leaf is S(b, t) .. S(b+1, t) of tile t, the rows
 // rowb[0, ntiles) and rowb[ntiles, 2 * ntiles), read from rows_lds (staged
-// there earlier) when given and ntiles <= kRowsLds.  Every wave reads the rows
-// of 64 tiles at a time, scans the run lengths itself (DPP) and copies the
-// runs of the tiles t = wave (mod waves), 64 dwords per instruction, to their
-// offsets in dst.  Complete after vm_wait_all() in every wave and a barrier.
+// there earlier) when `staged` and ntiles <= kRowsLds.  rows_lds always points
+// into LDS (also when not staged): a pointer that might be null or global
+// makes the compiler read it with flat loads, and a flat load waits for every
+// outstanding memory operation (the previous leaf's stores, the copies in
+// flight).  Every wave reads the rows of 64 tiles at a time, scans the run
+// lengths itself (DPP) and copies the runs of the tiles t = wave (mod waves),
+// 64 dwords per instruction, to their offsets in dst.  Complete after
+// vm_wait_all() in every wave and a barrier.
 template <class K, int NT>
 __device__ __forceinline__ void gather_to_lds_async(K* dst, const K* base, const uint32_t* rowb, uint32_t ntiles,
-                                                    const uint32_t* rows_lds, uint32_t tid) {
+                                                    const uint32_t* rows_lds, bool staged, uint32_t tid) {
     constexpr uint32_t NW = NT / 64, DW = sizeof(K) / 4;
     constexpr uint32_t kTile = TileShape<K>::kTileKeys;
     const uint32_t lane = tid & 63u, w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1924,7 +1936,7 @@ __device__ __forceinline__ void gather_to_lds_async(K* dst, const K* base, const
         const uint32_t t = c0 + lane;
         uint32_t s0 = 0, len = 0;
         if (t < ntiles) {
-            if (rows_lds && ntiles <= kRowsLds) {
+            if (staged && ntiles <= kRowsLds) {
                 s0 = rows_lds[t];
                 len = rows_lds[kRowsLds + t] - s0;
             } else {
@@ -1949,18 +1961,20 @@ __device__ __forceinline__ void gather_to_lds_async(K* dst, const K* base, const
     }
 }
 
-// Wave 0 stages the two rows of a gathered leaf of at most kRowsLds tiles in
-// rows_lds with asynchronous copies (complete after vm_wait_all() in wave 0
-// and a barrier).
+// Waves 0 and 1 stage the two rows of a gathered leaf of at most kRowsLds
+// tiles in rows_lds with asynchronous copies (complete after vm_wait_all() in
+// those waves and a barrier).
 __device__ __forceinline__ void stage_rows_async(uint32_t* rows_lds, const uint32_t* rowb, uint32_t ntiles,
                                                  uint32_t tid) {
-    if (tid < 64 && ntiles <= kRowsLds) {
-        const uint32_t lane = tid;
-        if (lane < ntiles) {
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowb + lane),
-                                             (__attribute__((address_space(3))) void*)(rows_lds), 4, 0, 0);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowb + ntiles + lane),
-                                             (__attribute__((address_space(3))) void*)(rows_lds + kRowsLds), 4, 0, 0);
+    static_assert(kRowsLds == 128, "two waves, one tile per lane");
+    if (tid < kRowsLds && ntiles <= kRowsLds) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);
+        if (w * 64u < ntiles && tid < ntiles) {
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowb + tid),
+                                             (__attribute__((address_space(3))) void*)(rows_lds + w * 64u), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rowb + ntiles + tid),
+                                             (__attribute__((address_space(3))) void*)(rows_lds + kRowsLds + w * 64u),
+                                             4, 0, 0);
         }
     }
 }
@@ -2133,10 +2147,10 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         return big_only(leaf_decode(a, list_mode, idx < count ? raw : none));
     };
     // the keys of leaf c into P: one contiguous range, or a bucket's runs in
-    // the tiles of a tile-local level (rows staged in rows_lds, or null)
-    auto issue_copy = [&](const LeafDesc& c, const uint32_t* rows_lds) {
+    // the tiles of a tile-local level (rows staged in rows_lds when staged)
+    auto issue_copy = [&](const LeafDesc& c, const uint32_t* rows_lds, bool staged) {
         if (c.gather)
-            gather_to_lds_async<K, NT>(P, (const K*)a.dn + c.src, a.lv.mat + c.row, c.ntiles, rows_lds, tid);
+            gather_to_lds_async<K, NT>(P, (const K*)a.dn + c.src, a.lv.mat + c.row, c.ntiles, rows_lds, staged, tid);
         else
             copy_to_lds_async<K, NT>(P, (c.from_dn ? (const K*)a.dn : (const K*)a.d) + c.start, c.n, tid);
     };
@@ -2151,7 +2165,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
     lds_sync();
     {
         const LeafDesc c = uniform_desc(sd[0]);
-        if (c.kind == kMine) issue_copy(c, nullptr);
+        if (c.kind == kMine) issue_copy(c, S.rows[0], false);
         const LeafDesc c1 = uniform_desc(sd[1]);
         if (c1.kind == kMine && c1.gather) stage_rows_async(S.rows[1], a.lv.mat + c1.row, c1.ntiles, tid);
     }
@@ -2218,7 +2232,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         if (tid < LL) hist[tid] = 0;
         if (tid == 0) s_nbig = 0;
         lds_sync();  // P consumed and A written out by every wave
-        if (nxt.kind == kMine) issue_copy(nxt, S.rows[(i + 1) % 2]);
+        if (nxt.kind == kMine) issue_copy(nxt, S.rows[(i + 1) % 2], true);
         fetch(g + 3 * G, (i + 1) % 2);  // decoded at the top of iteration i + 1
         if (cur.kind == 5) {
             // an oversize bucket of a tile-local level: its runs in order to
