@@ -53,6 +53,41 @@ __device__ __forceinline__ SegPlan uni(const SegPlan& p) {
 // Wait for every global load, store and global->LDS copy this wave issued.
 __device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Bounds-checked buffer access to [base, base + bytes): loads past the end
+// read 0 and stores past it are dropped by the hardware, so a partial tile
+// needs no branch around its loads and stores.  Every wave then issues the
+// same number of memory instructions on every path, and the compiler can
+// wait for a load by count (vmcnt(N)) instead of draining every later store.
+struct BufRange {
+    __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ BufRange buf_range(const void* base, uint32_t bytes) {
+    // dword 3: raw 32-bit data format, no swizzle (the gfx9 untyped-buffer setting)
+    return BufRange{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000)};
+}
+template <class K>
+__device__ __forceinline__ K buf_load(const BufRange& b, uint32_t byte_off) {
+    if constexpr (sizeof(K) == 4) {
+        return (K)__builtin_amdgcn_raw_buffer_load_b32(b.r, (int)byte_off, 0, 0);
+    } else {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)byte_off, 0, 0);
+        return (K)(((uint64_t)v.y << 32) | v.x);
+    }
+}
+template <class K>
+__device__ __forceinline__ void buf_store(K v, const BufRange& b, uint32_t byte_off) {
+    if constexpr (sizeof(K) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, b.r, (int)byte_off, 0, 0);
+    } else {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 w;
+        w.x = (uint32_t)v;
+        w.y = (uint32_t)((uint64_t)v >> 32);
+        __builtin_amdgcn_raw_buffer_store_b64(w, b.r, (int)byte_off, 0, 0);
+    }
+}
+
 // Inclusive scan over the 64 lanes of a wave with DPP: row_shr 1, 2, 4, 8
 // scan each row of 16 lanes (lanes shifted in from outside the row read 0),
 // then row_bcast:15 adds row r's last lane to row r + 1 (rows 1, 3) and
@@ -1414,7 +1449,9 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             }
             L.seg = seg_i;
         }
-        __syncthreads();  // the previous tile's stores have read stage
+        // LDS-only barriers in the loop: no wave waits for its stores of the
+        // previous tile (only for the loads whose registers it uses)
+        lds_sync();  // the previous tile's stores have read stage
         const uint32_t s = uni(L.seg);
         if (s != cur) {
             load_splitters<K>(a, s, L);
@@ -1449,15 +1486,16 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             if (J1.valid) run_table<K>(T, a, J1, J1.tb, rr_s0, rr_pre, ti, ntab);
         } else if (!have_keys) {
             K raw_keys[KPT];
+            const BufRange rin = buf_range(base + lo, cnt * (uint32_t)sizeof(K));
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) raw_keys[j] = base[min(lo + j * kScatterWG + ti, hi - 1u)];
+            for (int j = 0; j < KPT; ++j) raw_keys[j] = buf_load<K>(rin, (j * kScatterWG + ti) * (uint32_t)sizeof(K));
 #pragma unroll
             for (int j = 0; j < KPT; ++j) {
                 const K v = raw ? to_key<DT>(raw_keys[j]) : raw_keys[j];
                 keys[j] = (lo + j * kScatterWG + ti < hi) ? v : K(0);
             }
         }
-        __syncthreads();  // counters cleared (gathered input: run table complete)
+        lds_sync();  // counters cleared (gathered input: run table complete)
         if constexpr (kGather) {
             // the job and rows of the tile after next (loaded unconditionally:
             // a register assigned on one branch only costs a wait here).  When
@@ -1484,12 +1522,12 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
 #pragma unroll
             for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
         }
-        __syncthreads();  // every count is in hist
+        lds_sync();  // every count is in hist
         // in-tile bucket starts: thread b owns bucket b
         {
             const uint32_t mycnt = (ti < p.nbuckets) ? L.hist[ti] : 0u;
             uint32_t tot;
-            const uint32_t st = block_excl_scan<kScatterWG, false, false>(mycnt, wsum, &tot);
+            const uint32_t st = block_excl_scan<kScatterWG, true, false>(mycnt, wsum, &tot);
             uint32_t* col = a.mat + p.mat_base + tl;
             if (ti < p.nbuckets) {
                 L.hist[ti] = st;
@@ -1507,12 +1545,11 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         const uint32_t nlo = lo + kTile;
         const uint32_t nhi = next_tile ? min(nlo + kTile, seg.n) : 0u;
         K nxt[KPT];
+        // the next tile's range (empty without a next tile: every load reads 0)
+        const BufRange rnx = buf_range(base + (next_tile ? nlo : 0u), next_tile ? (nhi - nlo) * (uint32_t)sizeof(K) : 0u);
         if constexpr (kEarlyPrefetch) {
 #pragma unroll
-            for (int j = 0; j < KPT; ++j) {
-                const uint32_t idx = nlo + j * kScatterWG + ti;
-                nxt[j] = (idx < nhi) ? base[idx] : K(0);
-            }
+            for (int j = 0; j < KPT; ++j) nxt[j] = buf_load<K>(rnx, (j * kScatterWG + ti) * (uint32_t)sizeof(K));
         }
         if constexpr (kGather) {
             // the next tile's keys, through the run table built at the top
@@ -1538,27 +1575,23 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 }
             }
         }
-        // LDS-only barriers from here on: the next tile's keys stay in flight
-        // (the fence of __syncthreads() would wait for them)
         lds_sync();  // bucket starts in hist
 #pragma unroll
         for (int j = 0; j < KPT; ++j)
             if (slot_pos(j) < cnt) stage[L.hist[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
         if constexpr (!kEarlyPrefetch && !kGather) {
-            if (next_tile) {  // uniform
 #pragma unroll
-                for (int j = 0; j < KPT; ++j) nxt[j] = base[min(nlo + j * kScatterWG + ti, nhi - 1u)];
-            } else {
-#pragma unroll
-                for (int j = 0; j < KPT; ++j) nxt[j] = K(0);
-            }
+            for (int j = 0; j < KPT; ++j) nxt[j] = buf_load<K>(rnx, (j * kScatterWG + ti) * (uint32_t)sizeof(K));
         }
         lds_sync();  // the tile is staged
         // the tile back, grouped by bucket: coalesced stores of whole tiles
+        {
+            const BufRange rout = buf_range(dst + lo, cnt * (uint32_t)sizeof(K));
+            K sv[KPT];
 #pragma unroll
-        for (int j = 0; j < KPT; ++j) {
-            const uint32_t i = j * kScatterWG + ti;
-            if (i < cnt) dst[lo + i] = stage[i];
+            for (int j = 0; j < KPT; ++j) sv[j] = stage[j * kScatterWG + ti];
+#pragma unroll
+            for (int j = 0; j < KPT; ++j) buf_store<K>(sv[j], rout, (j * kScatterWG + ti) * (uint32_t)sizeof(K));
         }
         if constexpr (kGather) {
 #pragma unroll
