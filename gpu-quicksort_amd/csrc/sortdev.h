@@ -53,6 +53,24 @@ __device__ __forceinline__ SegPlan uni(const SegPlan& p) {
 // Wait for every global load, store and global->LDS copy this wave issued.
 __device__ __forceinline__ void vm_wait_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// A load of data no kernel running now writes (tables of an earlier launch),
+// through the constant address space: at a wave-uniform address it becomes a
+// scalar load, which the wave waits for on its own counter instead of behind
+// every vector load and store it has in flight.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p) {
+    typedef unsigned int w2 __attribute__((ext_vector_type(2)));
+    typedef unsigned int w4 __attribute__((ext_vector_type(4)));
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(4))) uint32_t*)p);
+    } else if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(4))) w2*)p);
+    } else {
+        static_assert(sizeof(T) == 16, "4, 8 or 16 bytes");
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(4))) w4*)p);
+    }
+}
+
 // Bounds-checked buffer access to [base, base + bytes): loads past the end
 // read 0 and stores past it are dropped by the hardware, so a partial tile
 // needs no branch around its loads and stores.  Every wave then issues the
@@ -1235,7 +1253,7 @@ template <class K>
 __device__ __forceinline__ void run_table(RunTab& T, const LevelArgs& a, const GatherJob& J, uint32_t tb,
                                           uint32_t s0, uint32_t pre, uint32_t tid, uint32_t ntab) {
     constexpr uint32_t kTile = TileShape<K>::kTileKeys;
-    const uint32_t base = uni(a.src_pre[J.g.row]);
+    const uint32_t base = ld_const(a.src_pre + J.g.row);
     const uint32_t t = tb + tid;
     const bool in = t < J.g.ntiles && tid < ntab;
     const uint32_t P = in ? pre - base : J.n;
@@ -1246,9 +1264,9 @@ __device__ __forceinline__ void run_table(RunTab& T, const LevelArgs& a, const G
     }
     const uint32_t le = (uint32_t)__popcll(__ballot(in && P <= J.hi));
     if (lane_id() == 0) T.wle[tid >> 6] = le;
+    const uint32_t te = tb + ntab;
+    const uint32_t Pe = te < J.g.ntiles ? ld_const(a.src_pre + J.g.row + te) - base : J.n;
     if (tid == 0) {
-        const uint32_t te = tb + ntab;
-        const uint32_t Pe = te < J.g.ntiles ? a.src_pre[J.g.row + te] - base : J.n;
         T.rpos[ntab] = min(max(Pe, J.lo), J.hi) - J.lo;
         T.more = Pe < J.hi ? 1u : 0u;
     }
@@ -1280,9 +1298,9 @@ __device__ __forceinline__ GatherJob next_job(const LevelArgs& a, const GatherJo
         N.tb = J.tb + nle - 1u;
     } else {
         N.seg = J.seg + 1;
-        N.g = uni(a.gather[N.seg]);
-        N.n = uni(a.segs[N.seg].n);
-        N.seg_end = N.seg + 1 < nseg ? uni(a.plans[N.seg + 1].tile_begin) : total;
+        N.g = ld_const(a.gather + N.seg);
+        N.n = ld_const(&a.segs[N.seg].n);
+        N.seg_end = N.seg + 1 < nseg ? ld_const(&a.plans[N.seg + 1].tile_begin) : total;
         N.lo = 0;
         N.tb = 0;
     }
