@@ -452,11 +452,12 @@ __global__ __launch_bounds__(256) void k_plan_gather(LevelArgs a) {
             const uint16_t m = a.meta[(size_t)s * kSplStride + i];
             if (m >> 8) eqi[(m & 0xFFu) + 1] = (uint16_t)(i + 1);
         }
+        // bucket starts in the segment from the scanned run lengths: row b
+        // of the segment begins after the keys of rows 0 .. b - 1 (row
+        // nbuckets, all zero, at the segment's size)
         const uint32_t b = threadIdx.x;
-        const uint32_t bs = (b < p.nbuckets) ? a.bsum[(size_t)s * kSplStride + b] : 0u;
-        uint32_t stot;
-        rsum[b] = block_excl_scan<256>(bs, wsum, &stot);
-        if (b == 0) rsum[p.nbuckets] = stot;
+        const uint32_t r0 = a.runlen[p.mat_base];
+        for (uint32_t i = b; i <= p.nbuckets; i += 256) rsum[i] = a.runlen[p.mat_base + (size_t)i * p.ntiles] - r0;
         __syncthreads();
         uint32_t rel = 0, size = 0, nfill = 0;
         if (b < p.nbuckets) {

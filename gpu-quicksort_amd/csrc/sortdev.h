@@ -1550,8 +1550,11 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             if (ti < p.nbuckets) {
                 L.hist[ti] = st;
                 col[(size_t)ti * p.ntiles] = st;
+                // bucket totals: added up here, or (in place, where every
+                // tile of one segment would add to the same counters) read
+                // from the scanned run lengths by the plan
                 if (a.tile_local == 2) a.runlen[p.mat_base + tl + (size_t)ti * p.ntiles] = mycnt;
-                if (mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
+                else if (mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
             } else if (ti == p.nbuckets) {
                 col[(size_t)ti * p.ntiles] = cnt;  // row nb: the tile's size
                 if (a.tile_local == 2) a.runlen[p.mat_base + tl + (size_t)ti * p.ntiles] = 0;
