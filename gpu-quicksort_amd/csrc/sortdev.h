@@ -1395,6 +1395,9 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     // tiles (slot tile & 1)
     __shared__ std::conditional_t<kGather, RunTab, uint32_t> T;
     __shared__ GatherJob s_job[kGather ? 2 : 1];
+    // the counters of odd tiles (even tiles use L.hist): a tile clears the
+    // next tile's counters, so no barrier separates the clear from the adds
+    __shared__ uint32_t hist_odd[kSplStride];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
@@ -1402,11 +1405,16 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, total);
     // run-table entries (one per thread; fewer only to drive the multi-table path)
     const uint32_t ntab = kGather ? min(max(a.run_tab, 64u), (uint32_t)kScatterWG) : 0u;
+    // every thread tracks the segment of its tile (wave-uniform, scalar loads)
     uint32_t seg_i = 0, seg_next = 0;
-    if (tid == 0 && t0 < t1) {
-        seg_i = upper_find(nseg, t0, [&](uint32_t i) { return a.plans[i].tile_begin; });
-        seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+    if (t0 < t1) {
+        seg_i = upper_find(nseg, t0, [&](uint32_t i) { return ld_const(&a.plans[i].tile_begin); });
+        seg_next = (seg_i + 1 < nseg) ? ld_const(&a.plans[seg_i + 1].tile_begin) : total;
     }
+    if (tid < (uint32_t)kSplStride) {
+        L.hist[tid] = 0;
+        hist_odd[tid] = 0;
+    }  // (the first tile's segment load ends with a barrier)
     uint32_t rr_s0 = 0, rr_pre = 0;  // gathered input: rows of the table of the tile after next
     K keys[KPT];
 #pragma unroll
@@ -1460,17 +1468,29 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         // recomputed per tile instead of being held across the loop)
         uint32_t ti;
         asm volatile("" : "=v"(ti) : "0"(tid));
-        if (ti == 0) {
+        if (!kGather || ti == 0) {
             while (t >= seg_next) {
                 ++seg_i;
-                seg_next = (seg_i + 1 < nseg) ? a.plans[seg_i + 1].tile_begin : total;
+                seg_next = (seg_i + 1 < nseg) ? ld_const(&a.plans[seg_i + 1].tile_begin) : total;
             }
-            L.seg = seg_i;
         }
-        // LDS-only barriers in the loop: no wave waits for its stores of the
-        // previous tile (only for the loads whose registers it uses)
-        lds_sync();  // the previous tile's stores have read stage
-        const uint32_t s = uni(L.seg);
+        if constexpr (kGather) {
+            if (ti == 0) L.seg = seg_i;
+        }
+        // No barrier opens a tile: every wave is past the previous tile's last
+        // barrier (staged), after which only the stores read LDS (stage); this
+        // tile writes stage after two more barriers, and its counters were
+        // cleared by the previous tile (gathered input: one barrier completes
+        // the run table built below).  The barriers in the loop are LDS-only:
+        // no wave waits for its stores (only for the loads whose registers it
+        // uses).
+        // (gathered input: measured faster with the previous schedule — a
+        // barrier opening the tile, one set of counters cleared behind it,
+        // and the barrier that completes the run table built below)
+        if constexpr (kGather) lds_sync();
+        uint32_t* const H = (kGather || !(t & 1)) ? L.hist : hist_odd;
+        uint32_t* const Hn = (kGather || (t & 1)) ? L.hist : hist_odd;
+        const uint32_t s = kGather ? uni(L.seg) : seg_i;
         if (s != cur) {
             load_splitters<K>(a, s, L);
             if (ti == 0) {
@@ -1483,7 +1503,9 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         const SegRec seg = uni(s_seg);
         const SegPlan p = uni(s_plan);
         const uint32_t tl = t - p.tile_begin;
-        if (ti < (uint32_t)kSplStride) L.hist[ti] = 0;
+        // the next tile's counters (last read before the staged barrier);
+        // gathered input: this tile's, behind the opening barrier
+        if (ti < (uint32_t)kSplStride) Hn[ti] = 0;
         const K* src = (seg.flags & SEG_SRC_DN) ? (const K*)a.dn : (const K*)a.d;
         // tile_local 2 partitions in place; the others write dn
         K* dst = (a.tile_local == 2 ? (K*)src : (K*)a.dn) + seg.start;
@@ -1513,13 +1535,14 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
                 keys[j] = (lo + j * kScatterWG + ti < hi) ? v : K(0);
             }
         }
-        lds_sync();  // counters cleared (gathered input: run table complete)
         if constexpr (kGather) {
-            // the job and rows of the tile after next (loaded unconditionally:
-            // a register assigned on one branch only costs a wait here).  When
-            // the next tile's runs need further tables, the tile after next
-            // starts from the first table's last tile (P <= hi still holds
-            // there).
+            lds_sync();  // run table complete
+            // the job and rows of the tile after next, requested before the
+            // classification (loaded unconditionally: a register assigned on
+            // one branch only costs a wait here).  When the next tile's runs
+            // need further tables, the tile after next starts from the first
+            // table's last tile (P <= hi still holds there).  s_job[t & 1] was
+            // last read before the previous tile's staged barrier.
             const GatherJob J1 = uni_job(s_job[(t + 1) & 1]);
             GatherJob J2 = J1;
             J2.valid = 0;
@@ -1536,19 +1559,19 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             classify_lut<K, KPT>(keys, bk, L, p);
 #pragma unroll
             for (int j = 0; j < KPT; ++j) valid[j] = slot_pos(j) < cnt;
-            hist_rank_n<KPT>(L.hist, bk, valid, rk);
+            hist_rank_n<KPT>(H, bk, valid, rk);
 #pragma unroll
             for (int j = 0; j < KPT; ++j) br[j] = (bk[j] << 16) | rk[j];
         }
         lds_sync();  // every count is in hist
         // in-tile bucket starts: thread b owns bucket b
         {
-            const uint32_t mycnt = (ti < p.nbuckets) ? L.hist[ti] : 0u;
+            const uint32_t mycnt = (ti < p.nbuckets) ? H[ti] : 0u;
             uint32_t tot;
             const uint32_t st = block_excl_scan<kScatterWG, true, false>(mycnt, wsum, &tot);
             uint32_t* col = a.mat + p.mat_base + tl;
             if (ti < p.nbuckets) {
-                L.hist[ti] = st;
+                H[ti] = st;
                 col[(size_t)ti * p.ntiles] = st;
                 // bucket totals: added up here, or (in place, where every
                 // tile of one segment would add to the same counters) read
@@ -1599,7 +1622,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         lds_sync();  // bucket starts in hist
 #pragma unroll
         for (int j = 0; j < KPT; ++j)
-            if (slot_pos(j) < cnt) stage[L.hist[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
+            if (slot_pos(j) < cnt) stage[H[br[j] >> 16] + (br[j] & 0xFFFFu)] = keys[j];
         if constexpr (!kEarlyPrefetch && !kGather) {
 #pragma unroll
             for (int j = 0; j < KPT; ++j) nxt[j] = buf_load<K>(rnx, (j * kScatterWG + ti) * (uint32_t)sizeof(K));
