@@ -2210,10 +2210,13 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
     // so no register holds a record in flight.
     auto fetch = [&](uint32_t idx, uint32_t slot) {
         if (tid < 8 && idx < count) {
-            const uint32_t* src =
-                (list_mode ? reinterpret_cast<const uint32_t*>(a.list) : reinterpret_cast<const uint32_t*>(a.lv.bkt)) +
-                (size_t)idx * 8;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + tid),
+            // a 32-bit offset from the scalar base, so that no 64-bit lane
+            // address stays live across the loop (one that did was spilled,
+            // and its reload waited for every copy in flight)
+            const uint32_t* base =
+                list_mode ? reinterpret_cast<const uint32_t*>(a.list) : reinterpret_cast<const uint32_t*>(a.lv.bkt);
+            const uint32_t o = idx * 8u + tid;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(base + o),
                                              (__attribute__((address_space(3))) void*)(S.recs[slot]), 4, 0, 0);
         }
     };
@@ -2275,7 +2278,35 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
                 pend.kind = 0;
             }
         };
+        // 6. (of g_{i-1}) buckets above the register capacity: one wave each
+        //    up to bitonic_cutoff, larger ones by the work-group.  Done here,
+        //    behind the barrier that opens the iteration, rather than behind
+        //    a barrier of their own: s_nbig holds every add of g_{i-1}'s
+        //    sort (thread 0 clears it after the next barrier).
+        auto finish_pending = [&]() {
+            const uint32_t nbig = s_nbig;
+            if (pend.kind && nbig) {  // uniform
+                const uint32_t n = pend.n;
+                uint32_t nb = 2;
+                while (nb < (uint32_t)LL && nb * a.sort_threshold < n) nb <<= 1;
+                for (uint32_t q = wave; q < nbig; q += NW) {
+                    const uint32_t b = bigs16[q];
+                    const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
+                    if (sz <= a.bitonic_cutoff) {
+                        K* As = A + lo;
+                        wave_sort_any<K>(sz, [&](uint32_t x) { return As[x]; }, [&](uint32_t x, K v) { As[x] = v; });
+                    }
+                }
+                lds_sync();
+                for (uint32_t q = 0; q < nbig; ++q) {
+                    const uint32_t b = bigs16[q];
+                    const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
+                    if (sz > a.bitonic_cutoff) lds_bitonic<K>(A + lo, sz, tid, NT);  // ends with a barrier
+                }
+            }
+        };
         if (g >= count) {
+            finish_pending();
             write_out_pending();
             break;
         }
@@ -2305,10 +2336,11 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
                 wmax[wave] = mx;
             }
         }
+        finish_pending();
         write_out_pending();
         if (tid < LL) hist[tid] = 0;
-        if (tid == 0) s_nbig = 0;
         lds_sync();  // P consumed and A written out by every wave
+        if (tid == 0) s_nbig = 0;  // (read by every wave before that barrier)
         if (nxt.kind == kMine) issue_copy(nxt, S.rows[(i + 1) % 2], true);
         fetch(g + 3 * G, (i + 1) % 2);  // decoded at the top of iteration i + 1
         if (cur.kind == 5) {
@@ -2392,7 +2424,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         // 3. bucket starts: thread t owns local bucket t
         const uint32_t mycnt = (tid < nb) ? hist[tid] : 0u;
         uint32_t tot;
-        const uint32_t mystart = block_excl_scan<NT, true>(mycnt, scanw, &tot);
+        const uint32_t mystart = block_excl_scan<NT, true, false>(mycnt, scanw, &tot);  // scanw is next written two barriers on
         if (tid < nb) bst[tid] = mystart;
         lds_sync();
         // 4. place keys by bucket
@@ -2427,27 +2459,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
             }
             if (!fits) bigs16[lds_add_rtn(&s_nbig, 1u)] = (uint16_t)tid;
         }
-        lds_sync();
-        // 6. buckets above the register capacity: one wave each up to
-        //    bitonic_cutoff, larger ones by the work-group
-        const uint32_t nbig = s_nbig;
-        if (nbig) {
-            for (uint32_t q = wave; q < nbig; q += NW) {
-                const uint32_t b = bigs16[q];
-                const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
-                if (sz <= a.bitonic_cutoff) {
-                    K* As = A + lo;
-                    wave_sort_any<K>(sz, [&](uint32_t x) { return As[x]; }, [&](uint32_t x, K v) { As[x] = v; });
-                }
-            }
-            lds_sync();
-            for (uint32_t q = 0; q < nbig; ++q) {
-                const uint32_t b = bigs16[q];
-                const uint32_t lo = bst[b], sz = (b + 1 < nb ? bst[b + 1] : n) - lo;
-                if (sz > a.bitonic_cutoff) lds_bitonic<K>(A + lo, sz, tid, NT);
-            }
-        }
-        pend = cur;  // written out at the top of the next iteration
+        pend = cur;  // finished (step 6) and written out in the next iteration
     }
 }
 
