@@ -56,7 +56,9 @@ def measured_traffic(kind, workload):
     total, found = 0.0, False
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and not line.startswith("#") and f[0] in KIND_KERNELS.get(kind, ()) and f[1] == workload:
+        # kernel names may carry their template arguments: k_tilepart<0,true>
+        if len(f) == 3 and not line.startswith("#") and f[0].split("<")[0] in KIND_KERNELS.get(kind, ()) \
+                and f[1] == workload:
             total += float(f[2])
             found = True
     return total if found else None

@@ -115,7 +115,12 @@ gqs_status gqs_autotune(gqs_ctx* c, int dtype, size_t n, int gen, uint32_t reps,
     for (const Axis& ax : axes) {
         if (status != GQS_OK) break;
         std::vector<uint32_t> vals = ax.values;
-        if (ax.ptr == &gqs_params::leaf_target) vals = {cur.leaf_max / 4, cur.leaf_max / 2};
+        // the fan-out is ceil(n / leaf_target): targets between a quarter and a
+        // half of leaf_max trade leaves above one work-group's capacity
+        // against half-empty ones
+        if (ax.ptr == &gqs_params::leaf_target)
+            vals = {cur.leaf_max / 4, cur.leaf_max * 5 / 16, cur.leaf_max * 3 / 8, cur.leaf_max * 7 / 16,
+                    cur.leaf_max / 2};
         for (uint32_t v : vals) {
             if (cur.*ax.ptr == v) continue;
             gqs_params cand = cur;

@@ -537,7 +537,7 @@ gqs_status gqs_default_params(int dtype, gqs_params* out) {
     gqs_params p{};
     p.mode = GQS_MODE_MULTIPIVOT;
     p.leaf_max = 2 * leaf_cap(dtype);
-    p.leaf_target = (dtype == DT_F64) ? 2048 : 4096;
+    p.leaf_target = (dtype == DT_F64) ? 3072 : 6144;  // 3/8 of leaf_max: buckets near 6 Ki (f64 3 Ki) keys
     p.oversample = 16;
     p.max_fanout = 256;
     p.max_levels = 8;

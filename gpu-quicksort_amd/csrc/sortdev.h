@@ -612,13 +612,18 @@ __global__ __launch_bounds__(kSplWG) void k_splitters(LevelArgs a) {
             }
             ncand = 2;
         } else {
-            uint32_t k = 2;
+            // fan-out: ceil(n / leaf_target) buckets (any count up to
+            // max_fanout, so that buckets land near leaf_target rather than
+            // between leaf_target / 2 and leaf_target).  The last level, which
+            // takes what the planned levels left above leaf_max (skewed or
+            // duplicate-heavy segments), splits as finely as it can: the most
+            // candidates, so that frequent values get equality buckets.
             const uint32_t want = (uint32_t)(((uint64_t)n + a.leaf_target - 1) / a.leaf_target);
-            while (k < want && k < a.max_fanout) k <<= 1;
+            uint32_t k = a.is_last ? a.max_fanout : max(2u, min(want, a.max_fanout));
             uint32_t S = 1;
             while (S < k * a.oversample && S < (uint32_t)kSampleMax) S <<= 1;
             while (S > n) S >>= 1;
-            while (k > 2 && k > S / 2) k >>= 1;
+            k = max(2u, min(k, S / 2));
             // stratified random sample: all of a thread's loads issue together
             constexpr uint32_t kPer = kSampleMax / kSplWG;
             K sv[kPer];
