@@ -1403,6 +1403,12 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     // the counters of odd tiles (even tiles use L.hist): a tile clears the
     // next tile's counters, so no barrier separates the clear from the adds
     __shared__ uint32_t hist_odd[kSplStride];
+    // A tile's count-matrix column (its run starts, and in place its run
+    // lengths) gathered over 16 tiles and stored a row segment at a time:
+    // stored per tile, the column was 257 scattered 4-byte stores (one cache
+    // line each, a quarter more write traffic than the keys' own)
+    constexpr uint32_t kCols = 16;
+    __shared__ uint32_t mcol[kGather ? 1 : 2][kSplStride + 1][kCols];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = a.ctl->total_tiles;
     const uint32_t nseg = min(a.ctl->nseg, a.seg_cap);
@@ -1574,18 +1580,21 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             const uint32_t mycnt = (ti < p.nbuckets) ? H[ti] : 0u;
             uint32_t tot;
             const uint32_t st = block_excl_scan<kScatterWG, true, false>(mycnt, wsum, &tot);
-            uint32_t* col = a.mat + p.mat_base + tl;
             if (ti < p.nbuckets) {
                 H[ti] = st;
-                col[(size_t)ti * p.ntiles] = st;
+                mcol[0][ti][tl % kCols] = st;
                 // bucket totals: added up here, or (in place, where every
                 // tile of one segment would add to the same counters) read
                 // from the scanned run lengths by the plan
-                if (a.tile_local == 2) a.runlen[p.mat_base + tl + (size_t)ti * p.ntiles] = mycnt;
-                else if (mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
+                if constexpr (!kGather) {
+                    if (a.tile_local == 2) mcol[kGather ? 0 : 1][ti][tl % kCols] = mycnt;
+                }
+                if (a.tile_local != 2 && mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
             } else if (ti == p.nbuckets) {
-                col[(size_t)ti * p.ntiles] = cnt;  // row nb: the tile's size
-                if (a.tile_local == 2) a.runlen[p.mat_base + tl + (size_t)ti * p.ntiles] = 0;
+                mcol[0][ti][tl % kCols] = cnt;  // row nb: the tile's size
+                if constexpr (!kGather) {
+                    if (a.tile_local == 2) mcol[kGather ? 0 : 1][ti][tl % kCols] = 0;
+                }
             }
         }
         // prefetch: the next tile when it is this work-group's and in the same
@@ -1633,6 +1642,26 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             for (int j = 0; j < KPT; ++j) nxt[j] = buf_load<K>(rnx, (j * kScatterWG + ti) * (uint32_t)sizeof(K));
         }
         lds_sync();  // the tile is staged
+        // the gathered columns out, after every kCols-th tile of the segment,
+        // its last tile and this work-group's last: row r's entries for the
+        // columns c0 .. tl are consecutive in the count matrix (before the
+        // tile's own stores, so the wait for the next tile's keys at the end
+        // of the iteration still counts those stores only)
+        if (tl % kCols == kCols - 1 || tl + 1 == p.ntiles || t + 1 == t1) {  // uniform
+            const uint32_t first = (t0 > p.tile_begin ? t0 - p.tile_begin : 0u);  // this work-group's first column
+            const uint32_t c0 = max(tl - tl % kCols, first);
+            const uint32_t nrow = p.nbuckets + 1;
+            for (uint32_t e = ti; e < nrow * kCols; e += kScatterWG) {
+                const uint32_t r = e / kCols, c = tl - tl % kCols + e % kCols;
+                if (c >= c0 && c <= tl) {
+                    const size_t at = p.mat_base + (size_t)r * p.ntiles + c;
+                    a.mat[at] = mcol[0][r][c % kCols];
+                    if constexpr (!kGather) {
+                        if (a.tile_local == 2) a.runlen[at] = mcol[kGather ? 0 : 1][r][c % kCols];
+                    }
+                }
+            }
+        }
         // the tile back, grouped by bucket: coalesced stores of whole tiles
         {
             const BufRange rout = buf_range(dst + lo, cnt * (uint32_t)sizeof(K));
