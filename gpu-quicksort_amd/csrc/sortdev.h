@@ -1474,6 +1474,7 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
     }
     uint32_t cur = 0xFFFFFFFFu;
     bool have_keys = false;  // keys[] already hold this tile's keys
+    uint32_t btot = 0;       // keys of bucket ti in this work-group's tiles of the segment so far
     for (uint32_t t = t0; t < t1; ++t) {
         // tid, opaque to the compiler (LDS addresses derived from it are
         // recomputed per tile instead of being held across the loop)
@@ -1583,13 +1584,12 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
             if (ti < p.nbuckets) {
                 H[ti] = st;
                 mcol[0][ti][tl % kCols] = st;
-                // bucket totals: added up here, or (in place, where every
-                // tile of one segment would add to the same counters) read
-                // from the scanned run lengths by the plan
+                // in place: the run lengths too (scanned, they give the
+                // runs' places in each bucket's key order)
                 if constexpr (!kGather) {
                     if (a.tile_local == 2) mcol[kGather ? 0 : 1][ti][tl % kCols] = mycnt;
                 }
-                if (a.tile_local != 2 && mycnt) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], mycnt);
+                btot += mycnt;
             } else if (ti == p.nbuckets) {
                 mcol[0][ti][tl % kCols] = cnt;  // row nb: the tile's size
                 if constexpr (!kGather) {
@@ -1647,6 +1647,13 @@ __global__ __launch_bounds__(kScatterWG, 4) void k_tilepart(LevelArgs a) {
         // columns c0 .. tl are consecutive in the count matrix (before the
         // tile's own stores, so the wait for the next tile's keys at the end
         // of the iteration still counts those stores only)
+        // bucket totals: one add per bucket for all of this work-group's
+        // tiles of the segment (in place, the plan reads them from the
+        // scanned run lengths instead)
+        if (tl + 1 == p.ntiles || t + 1 == t1) {
+            if (a.tile_local != 2 && ti < p.nbuckets && btot) atomicAdd(&a.bsum[(size_t)s * kSplStride + ti], btot);
+            btot = 0;
+        }
         if (tl % kCols == kCols - 1 || tl + 1 == p.ntiles || t + 1 == t1) {  // uniform
             const uint32_t first = (t0 > p.tile_begin ? t0 - p.tile_begin : 0u);  // this work-group's first column
             const uint32_t c0 = max(tl - tl % kCols, first);
