@@ -2004,7 +2004,9 @@ __device__ __forceinline__ LeafDesc leaf_decode(const LeafArgs& a, bool list_mod
         ds.src = (uint64_t)r.hi.x | ((uint64_t)r.hi.y << 32);
         ds.row = r.hi.z;
         ds.ntiles = r.hi.w;
-        if (gather) ds.kind = n <= a.leaf_cap ? 2u : (n <= a.leaf_max ? 4u : 5u);
+        // a gathered bucket above leaf_max is the next level's segment (the
+        // leaf pass only copies it together), even below leaf_cap
+        if (gather) ds.kind = n > a.leaf_max ? 5u : (n <= a.leaf_cap ? 2u : 4u);
         else ds.kind = (n <= a.bitonic_cutoff) ? 1u : (n <= a.leaf_cap ? 2u : (n <= a.leaf_max ? 4u : 3u));
     }
     return ds;

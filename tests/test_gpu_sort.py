@@ -122,6 +122,24 @@ def test_tile_local_rescue(gpu_ctx):
         gpu_ctx.set_params(gqsort.default_params())
 
 
+@pytest.mark.parametrize("dt", [gqsort.F32, gqsort.F64])
+def test_tile_local_rescue_float_keys(gpu_ctx, dt):
+    """The rescue path with keys whose sort representation differs from their
+    bits: the tile-local level's buckets above leaf_max (here below one
+    work-group's capacity too) are copied together for the next level in the
+    sort representation, and only sorted there, once."""
+    try:
+        for fanout, leaf_max in ((16, 2048), (32, 1024)):
+            a = gqsort.generate(dt, "normal", 14, 100000)
+            print(f"case tile-local rescue dt={dt} fanout={fanout} leaf_max={leaf_max}", flush=True)
+            got = gpu_sort(gpu_ctx, a, max_levels=2, leaf_max=leaf_max, leaf_target=32, max_fanout=fanout)
+            check(got, a, f"rescue dt={dt} fanout={fanout}")
+            s = gpu_ctx.stats()
+            assert s.levels_launched == 2 and s.levels_used == 2 and s.overflow == 0
+    finally:
+        gpu_ctx.set_params(gqsort.default_params())
+
+
 def test_unbounded_fallback(gpu_ctx):
     """With one level allowed, buckets above leaf_max take the unbounded local sort."""
     a = gqsort.generate(gqsort.U32, "uniform", 13, 100000)
