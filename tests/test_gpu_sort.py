@@ -140,6 +140,34 @@ def test_tile_local_rescue_float_keys(gpu_ctx, dt):
         gpu_ctx.set_params(gqsort.default_params())
 
 
+@pytest.mark.parametrize("dt", [gqsort.U32, gqsort.F32, gqsort.F64])
+@pytest.mark.parametrize("shift", [1, 3])
+def test_device_pointers_off_16_byte_boundary(gpu_ctx, dt, shift):
+    """d and dn anywhere a key may start (the contract only asks for the key
+    type's own alignment): the sort writes exactly [d, d + n), with 16-byte
+    stores only where they are aligned; the keys on either side stay."""
+    n = 300001
+    a = gqsort.generate(dt, "uniform" if dt == gqsort.U32 else "normal", 5, n)
+    kb = a.dtype.itemsize
+    lib = gpu_ctx._lib
+    st = gqsort.Stream(gpu_ctx)
+    d = gqsort.DeviceBuffer(gpu_ctx, (n + 8) * kb)
+    dn = gqsort.DeviceBuffer(gpu_ctx, (n + 8) * kb)
+    guard = np.full(n + 8, a[0], dtype=a.dtype)
+    guard[:shift] = a.max()
+    guard[shift + n:] = a.min()
+    guard[shift:shift + n] = a
+    d.upload(guard, st)
+    print(f"case off-boundary pointers dt={dt} shift={shift}", flush=True)
+    gpu_ctx.sort_device(dt, d.ptr + shift * kb, dn.ptr + (shift + 1) % 4 * kb, n, st)
+    st.sync()
+    got = d.download(a.dtype, n + 8, st)
+    st.sync()
+    check(got[shift:shift + n], a, f"off-boundary dt={dt} shift={shift}")
+    assert np.array_equal(got[:shift].view(np.uint8), guard[:shift].view(np.uint8))
+    assert np.array_equal(got[shift + n:].view(np.uint8), guard[shift + n:].view(np.uint8))
+
+
 def test_unbounded_fallback(gpu_ctx):
     """With one level allowed, buckets above leaf_max take the unbounded local sort."""
     a = gqsort.generate(gqsort.U32, "uniform", 13, 100000)
