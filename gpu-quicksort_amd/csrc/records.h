@@ -102,6 +102,8 @@ struct LevelCtl {
     uint32_t keys_lo;        // keys in this level's segments (low / high word)
     uint32_t keys_hi;
     uint32_t nhuge;          // leaves of this level in the large-leaf list
+    uint32_t leaf_next;      // next entry of the bucket table / leaf list for the leaf kernel to take
+    uint32_t huge_next;      // next entry of the large-leaf list to take
 };
 
 // One chunk of an equality bucket to fill in d with its key value (the

@@ -2219,6 +2219,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         uint32_t hist[LL + 1];
         uint32_t bst[LL + 1];
         uint32_t s_nbig;
+        uint32_t idx[4];    // leaf index of iteration i in idx[i % 4]
         uint32_t junk[64];  // targets of the cursor adds of lanes without a key, one per lane
         uint32_t scanw[NW];
         K wmin[NW], wmax[NW];
@@ -2239,8 +2240,6 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const bool list_mode = a.list != nullptr;
     const uint32_t count = list_mode ? min(*a.list_count, a.list_cap) : a.lv.ctl->total_buckets;
-    const uint32_t G = gridDim.x;
-    const uint32_t g0 = blockIdx.x;
     K* dout = (K*)a.d;
     auto big_only = [](LeafDesc ds) {
         // kinds sorted elsewhere: 1 (the wave kernel), 4 or 2 (the other
@@ -2280,11 +2279,23 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
     // Records are fetched three leaves ahead and decoded two ahead; a
     // gathered leaf's rows are staged in LDS one iteration before its keys
     // are copied, so no copy waits for a global load.
+    // Leaves are taken from the launch's counter (a work-group that draws
+    // large leaves takes fewer), three at first, then one per iteration:
+    // thread 0 claims the leaf of iteration i + 4 in iteration i and
+    // publishes it in the next one, so no wave waits for the atomic.
+    if (count == 0) return;  // (uniform) an empty pass takes no claims
+    uint32_t claim = 0;
     if (tid == 0) {
-        sd[0] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, g0, count)));
-        sd[1] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, g0 + G, count)));
+        const uint32_t b = atomicAdd(a.next, 3u);
+        S.idx[0] = b;
+        S.idx[1] = b + 1u;
+        S.idx[2] = b + 2u;
+        sd[0] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, b, count)));
+        sd[1] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, b + 1u, count)));
+        claim = atomicAdd(a.next, 1u);
     }
-    fetch(g0 + 2 * G, 0);
+    lds_sync();
+    fetch(S.idx[2], 0);
     lds_sync();
     {
         const LeafDesc c = uniform_desc(sd[0]);
@@ -2293,11 +2304,15 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         if (c1.kind == kMine && c1.gather) stage_rows_async(S.rows[1], a.lv.mat + c1.row, c1.ntiles, tid);
     }
     LeafDesc pend{0, 0, 0, 0, 0, 0, 0, 0, 0};  // sorted in A, not yet written out
-    uint32_t g = g0;
-    for (uint32_t i = 0;; ++i, g += G) {
-        vm_wait_all();  // keys of g_i in P, rows of g_{i+1}, record of g_{i+2}, stores of g_{i-2}
-        if (tid == 0) sd[(i + 2) % 4] = decode(g + 2 * G, i % 2);
+    for (uint32_t i = 0;; ++i) {
+        vm_wait_all();  // keys of g_i in P, rows of g_{i+1}, record of g_{i+2}, stores of g_{i-2}, the claim
+        if (tid == 0) {
+            S.idx[(i + 3) % 4] = claim;  // (idx[(i + 3) % 4] held g_{i-1}, read before the last barrier)
+            claim = atomicAdd(a.next, 1u);
+            sd[(i + 2) % 4] = decode(S.idx[(i + 2) % 4], i % 2);
+        }
         lds_sync();
+        const uint32_t g = uni(S.idx[i % 4]);  // claims only grow: once past count, always
         {
             // rows of g_{i+2}, staged while g_i is sorted (rows[i % 2] was
             // read by the copy of g_i, issued in the previous iteration)
@@ -2385,7 +2400,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         lds_sync();  // P consumed and A written out by every wave
         if (tid == 0) s_nbig = 0;  // (read by every wave before that barrier)
         if (nxt.kind == kMine) issue_copy(nxt, S.rows[(i + 1) % 2], true);
-        fetch(g + 3 * G, (i + 1) % 2);  // decoded at the top of iteration i + 1
+        fetch(uni(S.idx[(i + 3) % 4]), (i + 1) % 2);  // decoded at the top of iteration i + 1
         if (cur.kind == 5) {
             // an oversize bucket of a tile-local level: its runs in order to
             // d[start, start + n), where the next level partitions it again
