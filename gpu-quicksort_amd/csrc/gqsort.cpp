@@ -249,7 +249,7 @@ Grid Ctx::grid(const gqs_params& p) const {
     g.scan = cus * 2;
     g.plan = cus * 2;
     g.leaf = cus * (int)p.leaf_wgs_per_cu;
-    g.leaf_big = cus * 2;  // it takes its leaves from a counter: no more work-groups than fit
+    g.leaf_big = cus * 2;  // when it claims leaves from a counter: no more work-groups than fit
     g.huge = cus;
     g.tilepart = cus;
     return g;

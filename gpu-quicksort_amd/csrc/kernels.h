@@ -76,7 +76,7 @@ struct LeafArgs {
                                // ones (<= leaf_max) come from the large-leaf list
     const LeafRec* huge;       // the large-leaf list of this pass, count in *huge_count
     const uint32_t* huge_count;
-    uint32_t* next;            // the work counter of this launch (zeroed with the level's counters)
+    uint32_t* next;            // the work counter of this launch (zeroed with the level's counters), or null: static order
 };
 
 struct Grid {
@@ -84,7 +84,7 @@ struct Grid {
     int part;       // work-groups of count / scatter
     int scan;       // work-groups of the scan kernels
     int plan;       // work-groups of the plan kernel
-    int leaf;       // work-groups of the small-leaf kernel
+    int leaf;       // work-groups of the small-leaf kernel and of the leaf kernels in static order
     int leaf_big;   // work-groups of the 512-thread leaf kernel: as many as are resident (two per CU)
     int huge;       // work-groups of the large-leaf sort (one per CU)
     int tilepart;   // work-groups of the tile-local partition (one per CU)

@@ -29,17 +29,20 @@ template <int DT>
 hipError_t typed_leaf(const LeafArgs& a, const Grid& g, hipStream_t st) {
     hipLaunchKernelGGL((k_leaf_small<DT>), dim3(g.leaf), dim3(kLeafWG), 0, st, a);
     // gathered leaves (a tile-local level) are never sorted by one wave
+    // the leaves of a tile-local level are claimed from a counter by as many
+    // work-groups as are resident; others go by a static stride
+    const bool claimed = a.list == nullptr && a.lv.tile_local;
     if (a.leaf_max > a.bitonic_cutoff || a.is_last || a.lv.tile_local) {
         LeafArgs b = a;
-        b.next = &a.lv.ctl->leaf_next;
-        hipLaunchKernelGGL((k_leaf_big<DT, 512>), dim3(g.leaf_big), dim3(512), 0, st, b);
+        b.next = claimed ? &a.lv.ctl->leaf_next : nullptr;
+        hipLaunchKernelGGL((k_leaf_big<DT, 512>), dim3(claimed ? g.leaf_big : g.leaf), dim3(512), 0, st, b);
     }
     if (a.leaf_max > a.leaf_cap) {  // leaves above the 512-thread capacity: the large-leaf list
         LeafArgs h = a;
         h.list = a.huge;
         h.list_count = a.huge_count;
         h.list_cap = a.lv.huge_cap;
-        h.next = &a.lv.ctl->huge_next;
+        h.next = claimed ? &a.lv.ctl->huge_next : nullptr;
         hipLaunchKernelGGL((k_leaf_big<DT, 1024>), dim3(g.huge), dim3(1024), 0, st, h);
     }
     return hipGetLastError();

@@ -2279,20 +2279,27 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
     // Records are fetched three leaves ahead and decoded two ahead; a
     // gathered leaf's rows are staged in LDS one iteration before its keys
     // are copied, so no copy waits for a global load.
-    // Leaves are taken from the launch's counter (a work-group that draws
-    // large leaves takes fewer), three at first, then one per iteration:
-    // thread 0 claims the leaf of iteration i + 4 in iteration i and
-    // publishes it in the next one, so no wave waits for the atomic.
+    // Leaf order.  With a counter (a.next: the pass after a tile-local level,
+    // whose leaves are many and of similar size) leaves are claimed from it,
+    // so a work-group that draws large leaves takes fewer: three at first,
+    // then one per iteration, thread 0 claiming the leaf of iteration i + 4
+    // in iteration i and publishing it in the next one, so no wave waits for
+    // the atomic.  Without (leaf lists, contiguous levels: many small leaves,
+    // where one counter would be contended) work-group b takes leaves
+    // b, b + G, b + 2G, ... of an oversubscribed grid.
     if (count == 0) return;  // (uniform) an empty pass takes no claims
+    const bool claimed = a.next != nullptr;
+    const uint32_t G = gridDim.x;
     uint32_t claim = 0;
     if (tid == 0) {
-        const uint32_t b = atomicAdd(a.next, 3u);
+        const uint32_t b = claimed ? atomicAdd(a.next, 3u) : blockIdx.x;
+        const uint32_t step = claimed ? 1u : G;
         S.idx[0] = b;
-        S.idx[1] = b + 1u;
-        S.idx[2] = b + 2u;
+        S.idx[1] = b + step;
+        S.idx[2] = b + 2u * step;
         sd[0] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, b, count)));
-        sd[1] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, b + 1u, count)));
-        claim = atomicAdd(a.next, 1u);
+        sd[1] = big_only(leaf_decode(a, list_mode, leaf_rec(a, list_mode, b + step, count)));
+        claim = claimed ? atomicAdd(a.next, 1u) : b + 3u * G;
     }
     lds_sync();
     fetch(S.idx[2], 0);
@@ -2308,7 +2315,7 @@ __global__ __launch_bounds__(NTX, 4) void k_leaf_big(LeafArgs a) {
         vm_wait_all();  // keys of g_i in P, rows of g_{i+1}, record of g_{i+2}, stores of g_{i-2}, the claim
         if (tid == 0) {
             S.idx[(i + 3) % 4] = claim;  // (idx[(i + 3) % 4] held g_{i-1}, read before the last barrier)
-            claim = atomicAdd(a.next, 1u);
+            claim = claimed ? atomicAdd(a.next, 1u) : claim + G;
             sd[(i + 2) % 4] = decode(S.idx[(i + 2) % 4], i % 2);
         }
         lds_sync();

@@ -77,7 +77,7 @@ typedef struct gqs_params {
     uint32_t max_levels;   /* partition levels before every bucket goes to the local sort */
     uint32_t host_driven;  /* 1: read the level queue back after each level and stop early (CL12 loop) */
     uint32_t part_wgs_per_cu; /* persistent count/scatter work-groups per CU */
-    uint32_t leaf_wgs_per_cu; /* small-leaf kernel work-groups per CU (the 512-thread one: two per CU) */
+    uint32_t leaf_wgs_per_cu; /* leaf-kernel work-groups per CU (after a tile-local level: two per CU) */
     uint32_t sort_threshold;  /* local sort: bucket size its partition aims for (SORT_THRESHOLD role), 4..4096 */
     uint32_t bitonic_cutoff;  /* local sort: largest sub-bucket sorted by one wave's bitonic network, 64..512 */
 } gqs_params;
